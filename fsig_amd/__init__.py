@@ -1,0 +1,479 @@
+"""fsig_amd: MI355X (gfx950) subgraph-matching engine behind FSIG's matcher
+interface.
+
+The engine is native: ``libfsig_hip.so`` (HIP kernels + the C ABI of
+``include/fsig.h``) and ``libfsig_host.so`` (graph IO, CSR/PCSR/signature
+builders, seeded generators; ``include/fsig_host.h``).  This module binds both
+with ctypes for tests and benchmarks, and mirrors the reference's matcher
+classes (``GPUSubgraphMatcher``, ``MMatch``, ``GSI``; match/subgraph_match.h:
+39-81, match/MMatch.h:155-176, match/GSI.h:67-91) so that a test reads like a
+call of the reference's ``match()``.
+
+Nothing here falls back to a CPU path: if ``libfsig_hip.so`` is missing or no
+GPU is present, the GPU entry points raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+HIP_LIB_PATH = os.path.join(PKG_DIR, "libfsig_hip.so")
+HOST_LIB_PATH = os.path.join(PKG_DIR, "libfsig_host.so")
+FSIG_BIN = os.path.join(PKG_DIR, "bin", "FSIG")
+
+FSIG_OK, FSIG_EINVAL, FSIG_ENOMEM, FSIG_EHIP, FSIG_EUNSUPPORTED, FSIG_ENOTFOUND, FSIG_EAGAIN = 0, -1, -2, -3, -4, -5, -6
+ALG_MMATCH, ALG_GSI = 1, 2
+MODE_COUNT, MODE_ENUMERATE = 0, 1
+K_FILTER, K_REFINE, K_CEDGE, K_JOIN_COUNT, K_JOIN_WRITE, K_SCAN, K_BATCH = range(7)
+KERNEL_FAMILIES = ["filter", "refine", "cedge", "join_count", "join_write", "scan", "batch"]
+
+_u32p = C.POINTER(C.c_uint32)
+_i32p = C.POINTER(C.c_int32)
+_u64p = C.POINTER(C.c_uint64)
+_vp = C.c_void_p
+
+
+class FsigError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"fsig error {code}: {msg}")
+        self.code = code
+
+
+def _ptr(a: Optional[np.ndarray], t=_u32p):
+    if a is None:
+        return C.cast(None, t)
+    return a.ctypes.data_as(t)
+
+
+# ---------------------------------------------------------------------------
+# library loading
+_host = None
+_hip = None
+
+
+def host_lib():
+    """libfsig_host.so (CPU only)."""
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB_PATH):
+            raise FileNotFoundError(f"{HOST_LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(HOST_LIB_PATH)
+        L.fsig_hg_read.restype = _vp
+        L.fsig_hg_read.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int)]
+        L.fsig_hg_from_csr.restype = _vp
+        L.fsig_hg_from_csr.argtypes = [C.c_uint32, _u32p, _u32p, _i32p]
+        L.fsig_hg_from_edges.restype = _vp
+        L.fsig_hg_from_edges.argtypes = [C.c_uint32, C.c_uint64, _u32p, _i32p]
+        L.fsig_hg_chung_lu.restype = _vp
+        L.fsig_hg_chung_lu.argtypes = [C.c_uint32, C.c_uint64, C.c_double, C.c_uint64]
+        L.fsig_hg_rmat.restype = _vp
+        L.fsig_hg_rmat.argtypes = [C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_uint64]
+        L.fsig_hg_set_labels.argtypes = [_vp, C.c_int, C.c_int32, C.c_double, C.c_uint64]
+        L.fsig_hg_write.argtypes = [_vp, C.c_char_p]
+        L.fsig_hg_info.argtypes = [_vp, _u32p, _u64p, _i32p, _u32p]
+        L.fsig_hg_export.argtypes = [_vp, _u32p, _u32p, _i32p]
+        L.fsig_hg_inverted_index.argtypes = [_vp, _u32p, _u32p]
+        L.fsig_hg_signature.argtypes = [_vp, C.c_int, C.c_int, _u32p]
+        L.fsig_hg_free.argtypes = [_vp]
+        L.fsig_hg_walk_query.argtypes = [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int,
+                                         C.c_uint32, _i32p, _u32p, _u32p]
+        L.fsig_pcsr_build.restype = _vp
+        L.fsig_pcsr_build.argtypes = [_vp, C.c_int]
+        L.fsig_pcsr_num_units.restype = C.c_uint32
+        L.fsig_pcsr_num_units.argtypes = [_vp]
+        L.fsig_pcsr_unit_info.argtypes = [_vp, C.c_uint32, _u32p, _u32p]
+        L.fsig_pcsr_unit_offset.restype = _u32p
+        L.fsig_pcsr_unit_offset.argtypes = [_vp, C.c_uint32]
+        L.fsig_pcsr_unit_edges.restype = _u32p
+        L.fsig_pcsr_unit_edges.argtypes = [_vp, C.c_uint32]
+        L.fsig_pcsr_free.argtypes = [_vp]
+        L.fsig_murmur2.restype = C.c_uint32
+        L.fsig_murmur2.argtypes = [C.c_void_p, C.c_int, C.c_uint32]
+        L.fsig_write_embeddings.argtypes = [C.c_char_p, C.c_uint32, C.c_uint64, _u32p, C.c_uint64]
+        _host = L
+    return _host
+
+
+HIP_SYMBOLS = [
+    "fsig_ctx_create", "fsig_ctx_destroy", "fsig_last_error", "fsig_graph_upload_csr",
+    "fsig_graph_upload_pcsr", "fsig_graph_free", "fsig_match", "fsig_set_profiling",
+    "fsig_stats_reset", "fsig_kernel_stats", "fsig_last_candidates", "fsig_last_peak_bytes",
+    "fsig_set_tune", "fsig_get_tune",
+]
+
+
+class Tune(C.Structure):
+    _fields_ = [("lanes_per_row", C.c_uint32), ("strategy", C.c_uint32), ("lds_tile", C.c_uint32),
+                ("refine_passes", C.c_uint32)]
+
+
+def hip_lib():
+    """libfsig_hip.so.  Loading does not touch the GPU."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB_PATH):
+            raise FileNotFoundError(f"{HIP_LIB_PATH} is not built; the HIP engine is required (no CPU fallback)")
+        L = C.CDLL(HIP_LIB_PATH)
+        L.fsig_ctx_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(_vp)]
+        L.fsig_ctx_destroy.argtypes = [_vp]
+        L.fsig_last_error.restype = C.c_char_p
+        L.fsig_last_error.argtypes = [_vp]
+        L.fsig_graph_upload_csr.argtypes = [_vp, C.c_uint32, _u32p, _u32p, _i32p, C.c_uint32, _u32p, _u32p,
+                                            C.POINTER(_vp)]
+        L.fsig_graph_upload_pcsr.argtypes = [_vp, _vp, C.c_uint32, _u32p, C.POINTER(_u32p), _u32p,
+                                             C.POINTER(_u32p), _u32p, C.c_int]
+        L.fsig_graph_free.argtypes = [_vp, _vp]
+        L.fsig_match.argtypes = [_vp, _vp, C.c_uint32, _i32p, _u32p, _u32p, _u32p, C.c_int, C.c_int,
+                                 C.c_uint64, _u64p, _u32p, C.c_uint64]
+        L.fsig_set_profiling.argtypes = [_vp, C.c_int]
+        L.fsig_stats_reset.argtypes = [_vp]
+        L.fsig_kernel_stats.argtypes = [_vp, C.c_int, _u64p, C.POINTER(C.c_double), _u64p, _u64p]
+        L.fsig_last_candidates.argtypes = [_vp, C.c_uint32, _u32p]
+        L.fsig_last_peak_bytes.restype = C.c_uint64
+        L.fsig_last_peak_bytes.argtypes = [_vp]
+        L.fsig_set_tune.argtypes = [_vp, C.POINTER(Tune)]
+        L.fsig_get_tune.argtypes = [_vp, C.POINTER(Tune)]
+        _optional_bindings(L)
+        _hip = L
+    return _hip
+
+
+def _optional_bindings(L):
+    """Batched and served entry points (see include/fsig.h)."""
+    if hasattr(L, "fsig_match_batch"):
+        L.fsig_match_batch.argtypes = [_vp, _vp, C.c_uint32, _u32p, C.POINTER(_i32p), C.POINTER(_u32p),
+                                       C.POINTER(_u32p), _u64p, C.POINTER(C.c_double)]
+
+
+# ---------------------------------------------------------------------------
+# host graphs and queries
+class HostGraph:
+    """A CSR data graph held by libfsig_host.so."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise FsigError(FSIG_EINVAL, "host graph construction failed")
+        self._h = handle
+        self._L = host_lib()
+        n, nnz, nl, md = C.c_uint32(), C.c_uint64(), C.c_int32(), C.c_uint32()
+        self._L.fsig_hg_info(self._h, C.byref(n), C.byref(nnz), C.byref(nl), C.byref(md))
+        self.n, self.nnz, self.num_labels, self.max_degree = n.value, nnz.value, nl.value, md.value
+        self._arrays = None
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._L.fsig_hg_free(self._h)
+        except Exception:
+            pass
+
+    # constructors
+    @classmethod
+    def read(cls, path: str, labeled: bool = True) -> "HostGraph":
+        st = C.c_int(0)
+        h = host_lib().fsig_hg_read(path.encode(), int(labeled), C.byref(st))
+        if not h:
+            raise FsigError(st.value, f"cannot read {path}")
+        return cls(h)
+
+    @classmethod
+    def from_edges(cls, n: int, edges, labels=None) -> "HostGraph":
+        e = np.ascontiguousarray(np.asarray(edges, dtype=np.uint32).reshape(-1, 2))
+        lab = None if labels is None else np.ascontiguousarray(np.asarray(labels, dtype=np.int32))
+        return cls(host_lib().fsig_hg_from_edges(n, len(e), _ptr(e), _ptr(lab, _i32p)))
+
+    @classmethod
+    def chung_lu(cls, n: int, m: int, gamma: float, seed: int) -> "HostGraph":
+        return cls(host_lib().fsig_hg_chung_lu(n, m, gamma, seed))
+
+    @classmethod
+    def rmat(cls, scale: int, m: int, a=0.57, b=0.19, c=0.19, seed: int = 1) -> "HostGraph":
+        return cls(host_lib().fsig_hg_rmat(scale, m, a, b, c, seed))
+
+    def set_labels(self, kind: str, num_labels: int, s: float = 1.2, seed: int = 1) -> "HostGraph":
+        self._L.fsig_hg_set_labels(self._h, 0 if kind == "uniform" else 1, num_labels, s, seed)
+        self.num_labels = num_labels
+        self._arrays = None
+        return self
+
+    def write(self, path: str):
+        rc = self._L.fsig_hg_write(self._h, path.encode())
+        if rc:
+            raise FsigError(rc, f"cannot write {path}")
+
+    # arrays
+    def arrays(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        if self._arrays is None:
+            off = np.empty(self.n + 1, np.uint32)
+            adj = np.empty(self.nnz, np.uint32)
+            lab = np.empty(self.n, np.int32)
+            self._L.fsig_hg_export(self._h, _ptr(off), _ptr(adj), _ptr(lab, _i32p))
+            self._arrays = (off, adj, lab)
+        return self._arrays
+
+    def inverted_index(self) -> Tuple[np.ndarray, np.ndarray]:
+        idx = np.empty(self.num_labels + 1, np.uint32)
+        lst = np.empty(self.n, np.uint32)
+        self._L.fsig_hg_inverted_index(self._h, _ptr(idx), _ptr(lst))
+        return idx, lst
+
+    def signature(self, column_oriented: bool = True, x86_shift: bool = True) -> np.ndarray:
+        out = np.empty(16 * self.n, np.uint32)
+        self._L.fsig_hg_signature(self._h, int(column_oriented), int(x86_shift), _ptr(out))
+        return out
+
+    def pcsr(self, reference_unit_ids: bool = False) -> "HostPCSR":
+        return HostPCSR(self._L.fsig_pcsr_build(self._h, int(reference_unit_ids)), reference_unit_ids)
+
+    def walk_query(self, k: int, min_edges: int = 0, max_edges: int = 10 ** 6, seed: int = 1, bias: int = 0,
+                   max_attempts: int = 100000) -> "Query":
+        max_e = min(max_edges, k * (k - 1) // 2)
+        lab = np.empty(k, np.int32)
+        ed = np.empty(2 * max(max_e, 1), np.uint32)
+        walk = np.empty(k, np.uint32)
+        ne = self._L.fsig_hg_walk_query(self._h, k, min_edges, max_e, seed, bias, max_attempts,
+                                        _ptr(lab, _i32p), _ptr(ed), _ptr(walk))
+        if ne < 0:
+            raise FsigError(FSIG_EINVAL, f"no walk query with k={k}, edges in [{min_edges},{max_e}]")
+        q = Query(k, lab, ed[: 2 * ne].reshape(-1, 2))
+        q.walk = walk
+        return q
+
+
+class HostPCSR:
+    def __init__(self, handle, reference_unit_ids: bool):
+        self._h = handle
+        self._L = host_lib()
+        self.reference_unit_ids = reference_unit_ids
+        self.num_units = self._L.fsig_pcsr_num_units(handle)
+
+    def __del__(self):
+        try:
+            self._L.fsig_pcsr_free(self._h)
+        except Exception:
+            pass
+
+    def unit(self, u: int) -> Tuple[np.ndarray, np.ndarray]:
+        keys, ne = C.c_uint32(), C.c_uint32()
+        self._L.fsig_pcsr_unit_info(self._h, u, C.byref(keys), C.byref(ne))
+        off = np.ctypeslib.as_array(self._L.fsig_pcsr_unit_offset(self._h, u), shape=(32 * keys.value,)) \
+            if keys.value else np.empty(0, np.uint32)
+        ed = np.ctypeslib.as_array(self._L.fsig_pcsr_unit_edges(self._h, u), shape=(ne.value,)) \
+            if ne.value else np.empty(0, np.uint32)
+        return off.copy(), ed.copy()
+
+
+@dataclass
+class Query:
+    """A query graph: k vertices, labels, undirected edge list (u, v)."""
+    k: int
+    labels: np.ndarray
+    edges: np.ndarray
+    walk: Optional[np.ndarray] = None
+
+    def __post_init__(self):
+        self.labels = np.ascontiguousarray(np.asarray(self.labels, dtype=np.int32))
+        self.edges = np.asarray(self.edges, dtype=np.uint32).reshape(-1, 2)
+
+    def adjm(self) -> np.ndarray:
+        m = np.zeros(self.k, np.uint32)
+        for a, b in self.edges:
+            if a != b:
+                m[a] |= np.uint32(1 << int(b))
+                m[b] |= np.uint32(1 << int(a))
+        return m
+
+    def csr(self) -> Tuple[np.ndarray, np.ndarray]:
+        m = self.adjm()
+        off = np.zeros(self.k + 1, np.uint32)
+        adj = []
+        for u in range(self.k):
+            nb = [w for w in range(self.k) if (int(m[u]) >> w) & 1]
+            adj.extend(nb)
+            off[u + 1] = len(adj)
+        return off, np.asarray(adj, dtype=np.uint32)
+
+    def host_graph(self) -> HostGraph:
+        return HostGraph.from_edges(self.k, self.edges, self.labels)
+
+    def signature(self) -> np.ndarray:
+        """Row-major [k][16] signature (graph_format_transformer.cc:217-281)."""
+        return self.host_graph().signature(column_oriented=False)
+
+    @property
+    def num_edges(self) -> int:
+        return int(len({(min(a, b), max(a, b)) for a, b in self.edges.tolist() if a != b}))
+
+
+# ---------------------------------------------------------------------------
+# device side
+class Context:
+    """fsig_ctx: device, stream and HBM arena (match/subgraph_match.h:51-80)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self._L = hip_lib()
+        h = _vp()
+        rc = self._L.fsig_ctx_create(device, hbm_budget, C.byref(h))
+        if rc:
+            raise FsigError(rc, f"cannot create a context on device {device}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.fsig_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc: int):
+        if rc:
+            raise FsigError(rc, self._L.fsig_last_error(self._h).decode())
+
+    def upload(self, g: HostGraph, pcsr: bool = False, reference_unit_ids: bool = False) -> "DeviceGraph":
+        return DeviceGraph(self, g, pcsr, reference_unit_ids)
+
+    def set_profiling(self, on: bool):
+        self.check(self._L.fsig_set_profiling(self._h, int(on)))
+
+    def reset_stats(self):
+        self.check(self._L.fsig_stats_reset(self._h))
+
+    def kernel_stats(self, family: int) -> dict:
+        la, by, un = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        ms = C.c_double()
+        self.check(self._L.fsig_kernel_stats(self._h, family, C.byref(la), C.byref(ms), C.byref(by), C.byref(un)))
+        return {"launches": la.value, "ms": ms.value, "alg_bytes": by.value, "units": un.value}
+
+    def set_tune(self, lanes_per_row=0, strategy=0, lds_tile=0, refine_passes=0):
+        t = Tune(lanes_per_row, strategy, lds_tile, refine_passes)
+        self.check(self._L.fsig_set_tune(self._h, C.byref(t)))
+
+    def get_tune(self) -> dict:
+        t = Tune()
+        self.check(self._L.fsig_get_tune(self._h, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in Tune._fields_}
+
+    def last_candidates(self, k: int) -> np.ndarray:
+        out = np.zeros(k, np.uint32)
+        self.check(self._L.fsig_last_candidates(self._h, k, _ptr(out)))
+        return out
+
+    def last_peak_bytes(self) -> int:
+        return int(self._L.fsig_last_peak_bytes(self._h))
+
+
+class DeviceGraph:
+    """A data graph resident in HBM (CSR, and PCSR + signatures when asked)."""
+
+    def __init__(self, ctx: Context, g: HostGraph, pcsr: bool, reference_unit_ids: bool):
+        self.ctx, self.host = ctx, g
+        off, adj, lab = g.arrays()
+        idx, lst = g.inverted_index()
+        h = _vp()
+        ctx.check(ctx._L.fsig_graph_upload_csr(ctx._h, g.n, _ptr(off), _ptr(adj), _ptr(lab, _i32p),
+                                               g.num_labels, _ptr(idx), _ptr(lst), C.byref(h)))
+        self._h = h
+        self.has_pcsr = False
+        if pcsr:
+            self.attach_pcsr(reference_unit_ids)
+
+    def attach_pcsr(self, reference_unit_ids: bool = False):
+        p = self.host.pcsr(reference_unit_ids)
+        U = p.num_units
+        keys = np.zeros(U, np.uint32)
+        ned = np.zeros(U, np.uint32)
+        offs, edges = [], []
+        for u in range(U):
+            o, e = p.unit(u)
+            keys[u], ned[u] = len(o) // 32, len(e)
+            offs.append(np.ascontiguousarray(o, dtype=np.uint32))
+            edges.append(np.ascontiguousarray(e, dtype=np.uint32))
+        op = (_u32p * U)(*[_ptr(o) for o in offs])
+        ep = (_u32p * U)(*[_ptr(e) for e in edges])
+        sig = self.host.signature(column_oriented=True)
+        self.ctx.check(self.ctx._L.fsig_graph_upload_pcsr(self.ctx._h, self._h, U, _ptr(keys), op, _ptr(ned),
+                                                          ep, _ptr(sig), int(reference_unit_ids)))
+        self.has_pcsr = True
+
+    def free(self):
+        if getattr(self, "_h", None):
+            self.ctx._L.fsig_graph_free(self.ctx._h, self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class MatchResult:
+    count: int
+    embeddings: Optional[np.ndarray] = None  # rows x k, query-vertex order
+    candidates: Optional[np.ndarray] = None
+
+
+def match(dg: DeviceGraph, q: Query, alg: int = ALG_MMATCH, mode: int = MODE_COUNT, max_embeddings: int = 0,
+          emb_cap: int = 0) -> MatchResult:
+    """One fsig_match call."""
+    ctx = dg.ctx
+    off, adj = q.csr()
+    sig = q.signature() if alg == ALG_GSI else None
+    cnt = C.c_uint64(0)
+    out = np.empty((emb_cap, q.k), np.uint32) if emb_cap else None
+    ctx.check(ctx._L.fsig_match(ctx._h, dg._h, q.k, _ptr(q.labels, _i32p), _ptr(off), _ptr(adj), _ptr(sig),
+                                alg, mode, max_embeddings, C.byref(cnt), _ptr(out), emb_cap))
+    res = MatchResult(cnt.value)
+    if out is not None:
+        res.embeddings = out[: min(cnt.value, emb_cap)]
+    res.candidates = ctx.last_candidates(q.k)
+    return res
+
+
+class GPUSubgraphMatcher:
+    """Mirror of the reference's abstract matcher (match/subgraph_match.h:39-81):
+    holds the query and data graph, ``match()`` runs on the device."""
+    alg = ALG_MMATCH
+
+    def __init__(self, query_graph: Query, data_graph: HostGraph, dev_id: int = 0, ctx: Context = None,
+                 device_graph: DeviceGraph = None):
+        self.query_graph_ = query_graph
+        self.data_graph_ = data_graph
+        self.dev_id_ = dev_id
+        self.ctx = ctx
+        self.dg = device_graph
+        self.num_embeddings_ = 0
+        self.embeddings_ = None
+
+    def init_GPU(self, dev_id: int, hbm_budget: int = 1 << 31):
+        if self.ctx is None:
+            self.ctx = Context(dev_id, hbm_budget)
+
+    def match(self, mode: int = MODE_ENUMERATE, max_embeddings: int = 0, emb_cap: int = 0) -> MatchResult:
+        self.init_GPU(self.dev_id_)
+        if self.dg is None:
+            self.dg = self.ctx.upload(self.data_graph_, pcsr=self.alg == ALG_GSI)
+        r = match(self.dg, self.query_graph_, self.alg, mode, max_embeddings, emb_cap)
+        self.num_embeddings_ = r.count
+        self.embeddings_ = r.embeddings
+        return r
+
+
+class MMatch(GPUSubgraphMatcher):
+    """Alg 1 (match/MMatch.h:75-230)."""
+    alg = ALG_MMATCH
+
+
+class GSI(GPUSubgraphMatcher):
+    """Alg 2 (match/GSI.h:47-101)."""
+    alg = ALG_GSI

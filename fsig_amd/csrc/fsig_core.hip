@@ -1,0 +1,376 @@
+// libfsig_hip.so: context, arena, resident graph, device scan and the C ABI
+// entry points of include/fsig.h.
+#include <algorithm>
+#include <cstring>
+
+#include "device.h"
+#include "internal.h"
+
+int set_err(fsig_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+// ---------------------------------------------------------------------------
+// Device-wide exclusive scan (u32 in, u64 out): per-tile sums, one-workgroup
+// scan of the tile sums, then per-tile scans.  Tile = 256 threads x 4 items.
+namespace {
+
+constexpr uint32_t kTile = FSIG_BLOCK * 4;
+
+__global__ void __launch_bounds__(FSIG_BLOCK) k_scan_tile_sums(const uint32_t* __restrict__ in, uint64_t n,
+                                                               uint64_t* __restrict__ sums) {
+  __shared__ uint64_t lds[4];
+  const uint64_t base = (uint64_t)blockIdx.x * kTile + threadIdx.x * 4ull;
+  uint64_t s = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+    if (base + j < n) s += in[base + j];
+  s = block_sum(s, lds);
+  if (threadIdx.x == 0) sums[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(FSIG_BLOCK) k_scan_tile_offsets(uint64_t* __restrict__ sums, uint64_t ntiles,
+                                                                  uint64_t* __restrict__ total_out) {
+  __shared__ uint64_t lds[4];
+  uint64_t carry = 0;
+  for (uint64_t b = 0; b < ntiles; b += FSIG_BLOCK) {
+    uint64_t i = b + threadIdx.x;
+    uint64_t v = i < ntiles ? sums[i] : 0;
+    uint64_t tot;
+    uint64_t ex = block_exclusive_scan(v, lds, tot);
+    if (i < ntiles) sums[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ void __launch_bounds__(FSIG_BLOCK) k_scan_tiles(const uint32_t* __restrict__ in, uint64_t n,
+                                                           const uint64_t* __restrict__ tile_off,
+                                                           uint64_t* __restrict__ out) {
+  __shared__ uint64_t lds[4];
+  const uint64_t base = (uint64_t)blockIdx.x * kTile + threadIdx.x * 4ull;
+  uint32_t v[4];
+  uint64_t s = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    v[j] = base + j < n ? in[base + j] : 0;
+    s += v[j];
+  }
+  uint64_t tot;
+  uint64_t ex = block_exclusive_scan(s, lds, tot) + tile_off[blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    if (base + j < n) out[base + j] = ex;
+    ex += v[j];
+  }
+}
+
+}  // namespace
+
+int scan_u32_to_u64(fsig_ctx* c, const uint32_t* in, uint64_t* out, uint64_t n) {
+  KTimer t(c, FSIG_K_SCAN);
+  const uint64_t ntiles = (n + kTile - 1) / kTile;
+  size_t mk = c->arena.mark();
+  uint64_t* sums = (uint64_t*)c->arena.alloc(sizeof(uint64_t) * std::max<uint64_t>(ntiles, 1));
+  if (!sums) return set_err(c, FSIG_ENOMEM, "scan: arena exhausted");
+  if (ntiles > 0) {
+    k_scan_tile_sums<<<(uint32_t)ntiles, FSIG_BLOCK, 0, c->stream>>>(in, n, sums);
+  }
+  k_scan_tile_offsets<<<1, FSIG_BLOCK, 0, c->stream>>>(sums, ntiles, out + n);
+  if (ntiles > 0) {
+    k_scan_tiles<<<(uint32_t)ntiles, FSIG_BLOCK, 0, c->stream>>>(in, n, sums, out);
+  }
+  FSIG_HIP(c, hipGetLastError());
+  c->arena.release(mk);
+  t.done(n * 4 + n * 8 + 8, n);
+  return FSIG_OK;
+}
+
+int resolve_timings(fsig_ctx* c) {
+  for (size_t i = 0; i < c->timed_used; i++) {
+    float ms = 0;
+    FSIG_HIP(c, hipEventElapsedTime(&ms, c->timed[i].a, c->timed[i].b));
+    c->kstat[c->timed[i].fam].ms += ms;
+  }
+  c->timed_used = 0;
+  uint64_t fam[4 * FSIG_K_NUM];
+  int rc = read_u64(c, c->d_counters + 16, fam, 4 * FSIG_K_NUM);
+  if (rc) return rc;
+  for (int f = 0; f < FSIG_K_NUM; f++) {
+    c->kstat[f].alg_bytes += fam[4 * f];
+    c->kstat[f].units += fam[4 * f + 1];
+  }
+  FSIG_HIP(c, hipMemsetAsync(c->d_counters + 16, 0, 8 * 4 * FSIG_K_NUM, c->stream));
+  return FSIG_OK;
+}
+
+int read_u64(fsig_ctx* c, const void* dev, uint64_t* host, size_t count) {
+  if (count > 4096) return set_err(c, FSIG_EINVAL, "read_u64: too many words");
+  FSIG_HIP(c, hipMemcpyAsync(c->h_pinned, dev, count * 8, hipMemcpyDeviceToHost, c->stream));
+  FSIG_HIP(c, hipStreamSynchronize(c->stream));
+  std::memcpy(host, c->h_pinned, count * 8);
+  return FSIG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+extern "C" {
+
+int fsig_ctx_create(int device, uint64_t budget, fsig_ctx** out) {
+  if (!out) return FSIG_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return FSIG_EHIP;
+  if (device < 0 || device >= ndev) return FSIG_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return FSIG_EHIP;
+  fsig_ctx* c = new fsig_ctx();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+  size_t freeb = 0, totalb = 0;
+  if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) { delete c; return FSIG_EHIP; }
+  if (budget == 0) budget = (uint64_t)freeb / 4 * 3;
+  if (budget > freeb) { delete c; return FSIG_ENOMEM; }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess ||
+      hipMalloc((void**)&c->d_counters, 64 * sizeof(unsigned long long)) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_pinned, 4096 * 8, hipHostMallocDefault) != hipSuccess) {
+    fsig_ctx_destroy(c);
+    return FSIG_EHIP;
+  }
+  if (hipMalloc((void**)&c->arena.base, budget) != hipSuccess) {
+    fsig_ctx_destroy(c);
+    return FSIG_ENOMEM;
+  }
+  c->arena.cap = budget;
+  if (hipMemset(c->d_counters, 0, 64 * sizeof(unsigned long long)) != hipSuccess) {
+    fsig_ctx_destroy(c);
+    return FSIG_EHIP;
+  }
+  *out = c;
+  return FSIG_OK;
+}
+
+void fsig_ctx_destroy(fsig_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->arena.base) (void)hipFree(c->arena.base);
+  if (c->d_counters) (void)hipFree(c->d_counters);
+  if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+  for (auto& t : c->timed) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  if (c->ev_a) (void)hipEventDestroy(c->ev_a);
+  if (c->ev_b) (void)hipEventDestroy(c->ev_b);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* fsig_last_error(const fsig_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int fsig_graph_upload_csr(fsig_ctx* c, uint32_t n, const uint32_t* off, const uint32_t* adj,
+                          const int32_t* labels, uint32_t num_labels, const uint32_t* inv_index,
+                          const uint32_t* inv_list, fsig_graph** out) {
+  if (!c || !out || !off || !labels || !inv_index || !inv_list || (off[n] > 0 && !adj))
+    return set_err(c, FSIG_EINVAL, "upload_csr: null argument");
+  *out = nullptr;
+  if (num_labels == 0 || num_labels > (1u << 20)) return set_err(c, FSIG_EINVAL, "upload_csr: bad label count");
+  // Host validation: the join treats adjacency lists as sets.
+  if (off[0] != 0) return set_err(c, FSIG_EINVAL, "upload_csr: off[0] != 0");
+  uint32_t maxdeg = 0;
+  for (uint32_t v = 0; v < n; v++) {
+    if (off[v + 1] < off[v]) return set_err(c, FSIG_EINVAL, "upload_csr: offsets decrease");
+    maxdeg = std::max(maxdeg, off[v + 1] - off[v]);
+    if (labels[v] < 0 || (uint32_t)labels[v] >= num_labels)
+      return set_err(c, FSIG_EINVAL, "upload_csr: label out of range");
+    for (uint32_t i = off[v]; i < off[v + 1]; i++) {
+      if (adj[i] >= n) return set_err(c, FSIG_EINVAL, "upload_csr: neighbour id out of range");
+      if (adj[i] == v) return set_err(c, FSIG_EINVAL, "upload_csr: self-loop");
+      if (i > off[v] && adj[i] <= adj[i - 1])
+        return set_err(c, FSIG_EINVAL, "upload_csr: adjacency not strictly ascending");
+    }
+  }
+  if (inv_index[0] != 0 || inv_index[num_labels] != n)
+    return set_err(c, FSIG_EINVAL, "upload_csr: inverted index does not cover the vertices");
+  for (uint32_t l = 0; l < num_labels; l++) {
+    if (inv_index[l + 1] < inv_index[l]) return set_err(c, FSIG_EINVAL, "upload_csr: inverted index decreases");
+    for (uint32_t i = inv_index[l]; i < inv_index[l + 1]; i++) {
+      uint32_t v = inv_list[i];
+      if (v >= n || labels[v] != (int32_t)l || (i > inv_index[l] && v <= inv_list[i - 1]))
+        return set_err(c, FSIG_EINVAL, "upload_csr: inverted index inconsistent with labels");
+    }
+  }
+  FSIG_HIP(c, hipSetDevice(c->device));
+  fsig_graph* g = new fsig_graph();
+  g->n = n;
+  g->num_labels = num_labels;
+  g->nnz = off[n];
+  g->max_degree = maxdeg;
+  g->h_inv_index.assign(inv_index, inv_index + num_labels + 1);
+  auto up = [&](void** dst, const void* src, size_t bytes) -> bool {
+    if (hipMalloc(dst, std::max<size_t>(bytes, 4)) != hipSuccess) return false;
+    return bytes == 0 || hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  bool ok = up((void**)&g->d_off, off, sizeof(uint32_t) * (n + 1ull)) &&
+            up((void**)&g->d_adj, adj, sizeof(uint32_t) * g->nnz) &&
+            up((void**)&g->d_labels, labels, sizeof(int32_t) * n) &&
+            up((void**)&g->d_inv_index, inv_index, sizeof(uint32_t) * (num_labels + 1ull)) &&
+            up((void**)&g->d_inv_list, inv_list, sizeof(uint32_t) * n);
+  if (!ok) {
+    fsig_graph_free(c, g);
+    return set_err(c, FSIG_ENOMEM, "upload_csr: device allocation or copy failed");
+  }
+  *out = g;
+  return FSIG_OK;
+}
+
+int fsig_graph_upload_pcsr(fsig_ctx* c, fsig_graph* g, uint32_t num_units, const uint32_t* unit_keys,
+                           const uint32_t* const* unit_offset, const uint32_t* unit_nedges,
+                           const uint32_t* const* unit_edges, const uint32_t* sig, int ref_ids) {
+  if (!c || !g || !unit_keys || !unit_offset || !unit_nedges || !unit_edges || !sig)
+    return set_err(c, FSIG_EINVAL, "upload_pcsr: null argument");
+  const uint64_t L = g->num_labels;
+  if (num_units != L * (L + 1) / 2) return set_err(c, FSIG_EINVAL, "upload_pcsr: unit count != L(L+1)/2");
+  std::vector<UnitMeta> meta(num_units);
+  uint64_t gw = 0, ew = 0;
+  for (uint32_t u = 0; u < num_units; u++) {
+    meta[u].group_base = gw;
+    meta[u].edge_base = ew;
+    meta[u].keys = unit_keys[u];
+    meta[u].pad = 0;
+    gw += 32ull * unit_keys[u];
+    ew += unit_nedges[u];
+    const uint32_t* o = unit_offset[u];
+    // every stored range must lie inside the unit's edge array
+    for (uint64_t j = 0; j < unit_keys[u]; j++) {
+      for (int s = 0; s < 15; s++) {
+        if (o[32 * j + 2 * s] == FSIG_INVALID) break;
+        uint32_t b = o[32 * j + 2 * s + 1], e = o[32 * j + 2 * s + 3];
+        if (o[32 * j + 2 * s] >= g->n || b > e || e > unit_nedges[u])
+          return set_err(c, FSIG_EINVAL, "upload_pcsr: bucket slot out of range");
+      }
+      uint32_t nxt = o[32 * j + 30];
+      if (nxt != FSIG_INVALID && nxt >= unit_keys[u]) return set_err(c, FSIG_EINVAL, "upload_pcsr: bad overflow id");
+    }
+    for (uint32_t i = 0; i < unit_nedges[u]; i++)
+      if (unit_edges[u][i] >= g->n) return set_err(c, FSIG_EINVAL, "upload_pcsr: neighbour id out of range");
+  }
+  FSIG_HIP(c, hipSetDevice(c->device));
+  if (g->d_groups) { (void)hipFree(g->d_groups); g->d_groups = nullptr; }
+  if (g->d_pedges) { (void)hipFree(g->d_pedges); g->d_pedges = nullptr; }
+  if (g->d_units) { (void)hipFree(g->d_units); g->d_units = nullptr; }
+  if (g->d_sig) { (void)hipFree(g->d_sig); g->d_sig = nullptr; }
+  if (hipMalloc((void**)&g->d_groups, std::max<uint64_t>(gw, 1) * 4) != hipSuccess ||
+      hipMalloc((void**)&g->d_pedges, std::max<uint64_t>(ew, 1) * 4) != hipSuccess ||
+      hipMalloc((void**)&g->d_units, sizeof(UnitMeta) * std::max<uint32_t>(num_units, 1)) != hipSuccess ||
+      hipMalloc((void**)&g->d_sig, 64ull * std::max<uint32_t>(g->n, 1)) != hipSuccess)
+    return set_err(c, FSIG_ENOMEM, "upload_pcsr: device allocation failed");
+  for (uint32_t u = 0; u < num_units; u++) {
+    if (unit_keys[u])
+      FSIG_HIP(c, hipMemcpy(g->d_groups + meta[u].group_base, unit_offset[u], 128ull * unit_keys[u],
+                            hipMemcpyHostToDevice));
+    if (unit_nedges[u])
+      FSIG_HIP(c, hipMemcpy(g->d_pedges + meta[u].edge_base, unit_edges[u], 4ull * unit_nedges[u],
+                            hipMemcpyHostToDevice));
+  }
+  FSIG_HIP(c, hipMemcpy(g->d_units, meta.data(), sizeof(UnitMeta) * num_units, hipMemcpyHostToDevice));
+  FSIG_HIP(c, hipMemcpy(g->d_sig, sig, 64ull * g->n, hipMemcpyHostToDevice));
+  g->h_units = meta;
+  g->num_units = num_units;
+  g->ref_unit_ids = ref_ids;
+  g->has_pcsr = true;
+  return FSIG_OK;
+}
+
+void fsig_graph_free(fsig_ctx* c, fsig_graph* g) {
+  if (!g) return;
+  if (c) (void)hipSetDevice(c->device);
+  void* ptrs[] = {g->d_off, g->d_adj, g->d_labels, g->d_inv_index, g->d_inv_list,
+                  g->d_groups, g->d_pedges, g->d_units, g->d_sig};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete g;
+}
+
+int fsig_match(fsig_ctx* c, const fsig_graph* g, uint32_t k, const int32_t* qlabels, const uint32_t* qoff,
+               const uint32_t* qadj, const uint32_t* qsig, int alg, int mode, uint64_t max_emb,
+               uint64_t* count_out, uint32_t* emb_out, uint64_t emb_cap) {
+  if (!c) return FSIG_EINVAL;
+  if (!g || !qlabels || !qoff || !count_out) return set_err(c, FSIG_EINVAL, "match: null argument");
+  if (k > 0 && qoff[k] > 0 && !qadj) return set_err(c, FSIG_EINVAL, "match: null query adjacency");
+  if (mode != FSIG_MODE_COUNT && mode != FSIG_MODE_ENUMERATE) return set_err(c, FSIG_EINVAL, "match: bad mode");
+  if (alg != FSIG_ALG_MMATCH && alg != FSIG_ALG_GSI) return set_err(c, FSIG_EINVAL, "match: bad algorithm");
+  if (alg == FSIG_ALG_GSI && !g->has_pcsr) return set_err(c, FSIG_ENOTFOUND, "match: graph has no PCSR upload");
+  *count_out = 0;
+  FSIG_HIP(c, hipSetDevice(c->device));
+  QueryPlan P;
+  int rc = plan_classes(P, k, qlabels, qoff, qadj, qsig, alg, c->err);
+  if (rc) return rc;
+  for (uint32_t u = 0; u < k; u++)
+    if ((uint32_t)qlabels[u] >= g->num_labels) {
+      // a label absent from the data graph has no candidates
+      c->last_cand.assign(k, 0);
+      return FSIG_OK;
+    }
+  const size_t mk = c->arena.mark();
+  c->arena.peak = mk;
+  rc = alg == FSIG_ALG_MMATCH ? run_mmatch(c, g, P, mode, max_emb, count_out, emb_out, emb_cap)
+                              : run_gsi(c, g, P, mode, max_emb, count_out, emb_out, emb_cap);
+  c->last_peak = c->arena.peak - mk;
+  c->arena.release(mk);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return set_err(c, FSIG_EHIP, std::string("match: ") + hipGetErrorString(e));
+  int rt = resolve_timings(c);
+  return rc ? rc : rt;
+}
+
+int fsig_set_profiling(fsig_ctx* c, int on) {
+  if (!c) return FSIG_EINVAL;
+  c->profiling = on != 0;
+  return FSIG_OK;
+}
+
+int fsig_stats_reset(fsig_ctx* c) {
+  if (!c) return FSIG_EINVAL;
+  for (auto& s : c->kstat) s = KStat();
+  return FSIG_OK;
+}
+
+int fsig_kernel_stats(const fsig_ctx* c, int fam, uint64_t* launches, double* ms, uint64_t* bytes, uint64_t* units) {
+  if (!c || fam < 0 || fam >= FSIG_K_NUM) return FSIG_EINVAL;
+  const KStat& s = c->kstat[fam];
+  if (launches) *launches = s.launches;
+  if (ms) *ms = s.ms;
+  if (bytes) *bytes = s.alg_bytes;
+  if (units) *units = s.units;
+  return FSIG_OK;
+}
+
+int fsig_last_candidates(const fsig_ctx* c, uint32_t k, uint32_t* sizes) {
+  if (!c || !sizes) return FSIG_EINVAL;
+  for (uint32_t i = 0; i < k; i++) sizes[i] = i < c->last_cand.size() ? c->last_cand[i] : 0;
+  return FSIG_OK;
+}
+
+uint64_t fsig_last_peak_bytes(const fsig_ctx* c) { return c ? c->last_peak : 0; }
+
+int fsig_set_tune(fsig_ctx* c, const fsig_tune* t) {
+  if (!c || !t) return FSIG_EINVAL;
+  const uint32_t g = t->lanes_per_row;
+  if (g != 0 && g != 4 && g != 8 && g != 16 && g != 32 && g != 64) return set_err(c, FSIG_EINVAL, "tune: lanes_per_row");
+  if (t->strategy > 2) return set_err(c, FSIG_EINVAL, "tune: strategy");
+  c->tune = *t;
+  if (c->tune.refine_passes == 0) c->tune.refine_passes = 4;
+  return FSIG_OK;
+}
+
+int fsig_get_tune(const fsig_ctx* c, fsig_tune* t) {
+  if (!c || !t) return FSIG_EINVAL;
+  *t = c->tune;
+  return FSIG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,394 @@
+// Join (enumeration) kernels and the memory-blocked depth-first driver,
+// shared by alg 1 (candidate-edge CSR) and alg 2 (PCSR).
+//
+// A table at level p holds partial embeddings of the first p join-order
+// vertices, row-major, p u32 words per row.  Extending row r by the vertex w
+// = order[p] takes, for every backward neighbour b of w (query neighbour
+// earlier in the order), the sorted list of admissible data vertices for w
+// given phi(b); the children of r are the elements common to all those lists
+// that differ from every earlier data vertex of the same label.  This is the
+// per-row computation of the reference's precompute/compute/extend kernels
+// (match/MMatch_kernels.cuh:374-792) and of GSI's load/intersection/link
+// kernels (match/GSI_kernels.cuh:95-560).
+//
+// Work mapping (gfx950, wave64): G lanes of a wavefront share one row
+// (G in {4,8,16,32,64}).  The shortest list drives; each lane takes one
+// driver element per step and probes the other lists by binary search; a
+// wave ballot counts or places the survivors.  Lists never leave their
+// sorted order, so children come out sorted within a row.
+//
+// Levels are processed depth-first over chunks of children whose size is
+// bounded by the free arena (the reference's form_blocks,
+// memory_manager/memory_manager.h:220-242, and extend_embeddings_recursively,
+// match/MMatch.cu:1834-1985), but children are windows of a global child
+// numbering, so one huge row can also be split, and nothing spills to host.
+#pragma once
+
+#include <algorithm>
+
+#include "device.h"
+#include "internal.h"
+
+struct JoinLevel {
+  uint32_t p;                 // columns of the parent rows
+  uint32_t nb;                // backward neighbours
+  uint32_t ncf;               // conflict columns (same label, not adjacent)
+  uint32_t cfsame;            // bit j: conflict j compares row words directly
+  uint32_t any_cross;         // some conflict compares data vertex ids
+  uint32_t wsize;             // |C(w)|; the driver when nb == 0
+  uint32_t range_ids;         // nb == 0: driver element t is wcand[t], else t
+  uint32_t bcol[MAXQ];
+  uint32_t cfcol[MAXQ];
+  const uint32_t* cfcand[MAXQ];  // row word -> data vertex for cross conflicts
+  const uint32_t* wcand;         // candidate index -> data vertex of w
+};
+
+struct JoinIO {
+  const uint32_t* rows;
+  uint64_t row_lo, row_hi;    // rows [row_lo, row_hi) are processed
+  uint32_t* cnt;              // count pass: children per row (absolute row index), may be null
+  const uint64_t* off;        // write pass: first child index of each row (absolute)
+  uint32_t* out;              // write pass: child table, p+1 words per child
+  uint64_t out_lo, out_hi;    // write pass: children [out_lo, out_hi) are written, at out_lo -> out[0]
+  unsigned long long* total;  // count pass: += children (may be null)
+  unsigned long long* stats;  // [0] += algorithmic bytes, [1] += units (rows or children)
+};
+
+// Candidate-edge CSR lists (alg 1): list i of row value x (a candidate index of
+// the backward neighbour's class) is lst[i][off[i][x] .. off[i][x+1]), holding
+// candidate indices of w's class.
+struct CandEdgeSrc {
+  const uint32_t* off[MAXQ];
+  const uint32_t* lst[MAXQ];
+  __device__ __forceinline__ void range(uint32_t i, uint32_t x, uint32_t& s, uint32_t& n) const {
+    const uint32_t* o = off[i];
+    s = o[x];
+    n = o[x + 1] - s;
+  }
+  __device__ __forceinline__ const uint32_t* list(uint32_t i) const { return lst[i]; }
+  __device__ __forceinline__ bool member(uint32_t) const { return true; }
+};
+
+// PCSR lists (alg 2): list i of a data vertex x is x's neighbour list in the
+// unit of the label pair (L(w), L(b_i)), found by MurmurHash2 bucket probing
+// (the lookup of match/GSI_kernels.cuh:95-133 with a correct chain walk).
+// Elements are data vertex ids; membership in C(w) is a bit of `mask`.
+struct PcsrSrc {
+  const uint32_t* groups;
+  const uint32_t* edges;
+  const uint32_t* mask;
+  uint32_t wbit;
+  UnitMeta unit[MAXQ];
+  __device__ __forceinline__ void range(uint32_t i, uint32_t x, uint32_t& s, uint32_t& n) const {
+    const UnitMeta u = unit[i];
+    s = 0;
+    n = 0;
+    if (u.keys == 0) return;
+    uint32_t grp = murmur2_u32(x, 17u) % u.keys;
+    for (uint32_t hops = 0; hops <= u.keys && grp != FSIG_INVALID; hops++) {
+      const uint32_t* gw = groups + u.group_base + 32ull * grp;
+      for (uint32_t slot = 0; slot < 15; slot++) {
+        const uint32_t key = gw[2 * slot];
+        if (key == FSIG_INVALID) return;  // slots fill from 0; a hole ends the bucket
+        if (key == x) {  // list is edges[edge_base + start, edge_base + end)
+          s = gw[2 * slot + 1];
+          n = gw[2 * slot + 3] - s;
+          return;
+        }
+      }
+      grp = gw[30];
+    }
+  }
+  __device__ __forceinline__ const uint32_t* list(uint32_t i) const { return edges + unit[i].edge_base; }
+  __device__ __forceinline__ bool member(uint32_t x) const { return (mask[x] >> wbit) & 1u; }
+};
+
+template <int G>
+__device__ __forceinline__ uint32_t group_col(const uint32_t (&rv)[(MAXQ + G - 1) / G], uint32_t c) {
+  constexpr int NR = (MAXQ + G - 1) / G;
+  const uint32_t j = c / G;
+  uint32_t v = rv[0];
+#pragma unroll
+  for (int jj = 1; jj < NR; jj++)
+    if ((uint32_t)jj == j) v = rv[jj];
+  return __shfl(v, (int)(c % G), G);
+}
+
+// WRITE == false: count pass (children per row into io.cnt, total into
+// counters[0]).  WRITE == true: children of rows [row_lo, row_hi) whose global
+// index falls in [out_lo, out_hi) are written as p+1 word rows.
+template <int G, bool WRITE, class Src>
+__global__ void __launch_bounds__(FSIG_BLOCK) k_join(JoinLevel L, Src S, JoinIO io) {
+  constexpr int GPB = FSIG_BLOCK / G;  // groups per block
+  constexpr int NR = (MAXQ + G - 1) / G;
+  __shared__ uint32_t sh_s[GPB][MAXQ];
+  __shared__ uint32_t sh_n[GPB][MAXQ];
+  __shared__ uint32_t sh_cf[GPB][MAXQ];
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t gl = lane & (G - 1);
+  const uint32_t gw = threadIdx.x / G;
+  const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
+  const uint64_t below = (1ull << lane) - 1;
+  const uint64_t ngroups = (uint64_t)gridDim.x * GPB;
+  unsigned long long my_children = 0, my_bytes = 0, my_rows = 0;
+
+  for (uint64_t r = io.row_lo + (uint64_t)xcd_block() * GPB + gw; r < io.row_hi; r += ngroups) {
+    uint32_t rv[NR];
+    const uint32_t* row = io.rows + r * L.p;
+#pragma unroll
+    for (int j = 0; j < NR; j++) {
+      const uint32_t col = j * G + gl;
+      rv[j] = col < L.p ? row[col] : 0u;
+    }
+    // list ranges of the backward neighbours; the shortest one drives
+    uint32_t dlen = L.wsize, d = 0;
+    uint64_t elems = 0;
+    for (uint32_t i = 0; i < L.nb; i++) {
+      const uint32_t x = group_col<G>(rv, L.bcol[i]);
+      uint32_t s, n;
+      S.range(i, x, s, n);
+      sh_s[gw][i] = s;
+      sh_n[gw][i] = n;
+      elems += n;
+      if (i == 0 || n < dlen) { dlen = n; d = i; }
+    }
+    for (uint32_t j = 0; j < L.ncf; j++) {
+      uint32_t x = group_col<G>(rv, L.cfcol[j]);
+      if (!((L.cfsame >> j) & 1u)) x = L.cfcand[j][x];
+      sh_cf[gw][j] = x;
+    }
+    uint64_t base = 0;
+    uint64_t child0 = 0;
+    if (WRITE) child0 = io.off[r];
+    const uint32_t* dl = L.nb ? S.list(d) + sh_s[gw][d] : nullptr;
+    for (uint32_t t0 = 0; t0 < dlen; t0 += G) {
+      const uint32_t t = t0 + gl;
+      bool ok = t < dlen;
+      uint32_t x = 0;
+      if (ok) x = L.nb ? dl[t] : (L.range_ids ? L.wcand[t] : t);
+      for (uint32_t i = 0; i < L.nb && ok; i++) {
+        if (i == d) continue;
+        ok = sorted_contains(S.list(i) + sh_s[gw][i], sh_n[gw][i], x);
+      }
+      if (ok) ok = S.member(x);
+      if (ok && L.ncf) {
+        const uint32_t xid = L.any_cross ? L.wcand[x] : x;
+        for (uint32_t j = 0; j < L.ncf && ok; j++) {
+          const uint32_t cmp = ((L.cfsame >> j) & 1u) ? x : xid;
+          ok = cmp != sh_cf[gw][j];
+        }
+      }
+      const uint64_t bal = __ballot(ok) & gmask;
+      if (WRITE) {
+        const uint64_t gidx = child0 + base + __popcll(bal & below);
+        const bool put = ok && gidx >= io.out_lo && gidx < io.out_hi;
+        uint32_t* dst = io.out + (gidx - io.out_lo) * (L.p + 1);
+        for (uint32_t c = 0; c < L.p; c++) {
+          const uint32_t v = group_col<G>(rv, c);
+          if (put) dst[c] = v;
+        }
+        if (put) dst[L.p] = x;
+      }
+      base += __popcll(bal);
+    }
+    if (!WRITE && io.cnt && gl == 0) io.cnt[r] = (uint32_t)base;
+    if (gl == 0) {
+      // algorithmic bytes: row words, an offset pair per backward neighbour,
+      // each list element once; then a u32 count (count pass) or a u64
+      // offset and 4(p+1) bytes per child written (write pass)
+      uint64_t b = 4ull * L.p + 8ull * L.nb + 4ull * elems;
+      if (WRITE) {
+        const uint64_t lo = child0 > io.out_lo ? child0 : io.out_lo;
+        const uint64_t hi = (child0 + base) < io.out_hi ? child0 + base : io.out_hi;
+        const uint64_t put = hi > lo ? hi - lo : 0;
+        b += 8 + put * 4ull * (L.p + 1);
+        my_rows += put;
+      } else {
+        b += 4;
+        my_rows += 1;
+      }
+      my_children += base;
+      my_bytes += b;
+    }
+  }
+  my_children = wave_sum_u64(my_children);
+  my_bytes = wave_sum_u64(my_bytes);
+  my_rows = wave_sum_u64(my_rows);
+  if (lane == 0) {
+    if (!WRITE && io.total) atomicAdd(io.total, my_children);
+    if (io.stats) {
+      atomicAdd(&io.stats[0], my_bytes);
+      atomicAdd(&io.stats[1], my_rows);
+    }
+  }
+}
+
+// Finds, for child windows [w*cap, (w+1)*cap), the first and one-past-last
+// parent rows: lo = last r with off[r] <= w*cap, hi = first r with off[r] >=
+// min((w+1)*cap, total).  off has nrows+1 entries.
+static __global__ void k_window_rows(const uint64_t* __restrict__ off, uint64_t nrows, uint64_t cap,
+                                     uint64_t total, uint32_t nwin, unsigned long long* __restrict__ out) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwin) return;
+  const uint64_t lo = (uint64_t)w * cap;
+  const uint64_t hi = lo + cap < total ? lo + cap : total;
+  // first row: last r with off[r] <= lo, i.e. upper_bound(lo) - 1
+  uint64_t a = 0, b = nrows + 1;
+  while (a < b) {
+    uint64_t m = (a + b) >> 1;
+    if (off[m] <= lo) a = m + 1; else b = m;
+  }
+  out[2 * w] = a - 1;
+  // one past the last row: first r with off[r] >= hi
+  a = 0;
+  b = nrows + 1;
+  while (a < b) {
+    uint64_t m = (a + b) >> 1;
+    if (off[m] < hi) a = m + 1; else b = m;
+  }
+  out[2 * w + 1] = a > nrows ? nrows : a;
+}
+
+// Host launcher for the join kernel, dispatching on G.
+template <bool WRITE, class Src>
+int launch_join(fsig_ctx* c, uint32_t G, const JoinLevel& L, const Src& S, const JoinIO& io) {
+  const uint64_t nrows = io.row_hi - io.row_lo;
+  if (nrows == 0) return FSIG_OK;
+  const uint64_t gpb = FSIG_BLOCK / G;
+  uint64_t blocks = (nrows + gpb - 1) / gpb;
+  const uint64_t max_blocks = (uint64_t)c->num_cus * 16;  // grid-stride beyond this
+  blocks = std::min(blocks, max_blocks);
+  if (blocks >= 64) blocks &= ~7ull;  // whole multiples of 8 XCDs
+  const uint32_t grid = (uint32_t)std::max<uint64_t>(blocks, 1);
+  switch (G) {
+    case 4: k_join<4, WRITE, Src><<<grid, FSIG_BLOCK, 0, c->stream>>>(L, S, io); break;
+    case 8: k_join<8, WRITE, Src><<<grid, FSIG_BLOCK, 0, c->stream>>>(L, S, io); break;
+    case 16: k_join<16, WRITE, Src><<<grid, FSIG_BLOCK, 0, c->stream>>>(L, S, io); break;
+    case 32: k_join<32, WRITE, Src><<<grid, FSIG_BLOCK, 0, c->stream>>>(L, S, io); break;
+    default: k_join<64, WRITE, Src><<<grid, FSIG_BLOCK, 0, c->stream>>>(L, S, io); break;
+  }
+  FSIG_HIP(c, hipGetLastError());
+  return FSIG_OK;
+}
+
+// Receives completed embeddings (level k tables, join-order words).
+struct LeafSink {
+  virtual ~LeafSink() = default;
+  virtual int take(const uint32_t* rows, uint64_t n) = 0;
+};
+
+// Provider: supplies the level description and list source for each level:
+//   using SrcType = ...; JoinLevel level(uint32_t p); SrcType src(uint32_t p);
+template <class Provider>
+class JoinDriver {
+ public:
+  using Src = typename Provider::SrcType;
+  JoinDriver(fsig_ctx* c, Provider& pv, uint32_t k, uint32_t G, int mode, uint64_t max_emb, LeafSink* sink)
+      : c_(c), pv_(pv), k_(k), G_(G), mode_(mode), max_emb_(max_emb), sink_(sink) {}
+
+  uint64_t total() const { return total_; }
+
+  int run(const uint32_t* level1, uint64_t n1) { return expand(1, level1, n1); }
+
+ private:
+  bool bounded() const { return mode_ == FSIG_MODE_ENUMERATE && max_emb_ != 0; }
+  uint64_t remaining() const { return bounded() ? max_emb_ - total_ : ~0ull; }
+
+  int expand(uint32_t p, const uint32_t* rows, uint64_t R) {
+    if (R == 0) return FSIG_OK;
+    if (p == k_) {
+      uint64_t n = std::min<uint64_t>(R, remaining());
+      if (sink_ && n) {
+        int rc = sink_->take(rows, n);
+        if (rc) return rc;
+      }
+      total_ += n;
+      return FSIG_OK;
+    }
+    const size_t mk = c_->arena.mark();
+    const JoinLevel L = pv_.level(p);
+    const Src S = pv_.src(p);
+    const bool last = p + 1 == k_;
+    unsigned long long* res = c_->d_counters;  // [0] children of this launch
+    int rc;
+    if (last && mode_ == FSIG_MODE_COUNT) {
+      FSIG_HIP(c_, hipMemsetAsync(res, 0, sizeof(unsigned long long), c_->stream));
+      JoinIO io{rows, 0, R, nullptr, nullptr, nullptr, 0, 0, res, family_counters(c_, FSIG_K_JOIN_COUNT)};
+      KTimer t(c_, FSIG_K_JOIN_COUNT);
+      if ((rc = launch_join<false>(c_, G_, L, S, io))) return rc;
+      t.done(0, 0);
+      uint64_t h;
+      if ((rc = read_u64(c_, res, &h, 1))) return rc;
+      total_ += h;
+      return FSIG_OK;
+    }
+    uint32_t* cnt = (uint32_t*)c_->arena.alloc(4 * R);
+    uint64_t* off = (uint64_t*)c_->arena.alloc(8 * (R + 1));
+    if (!cnt || !off) return set_err(c_, FSIG_ENOMEM, "join: arena exhausted for row counts");
+    {
+      JoinIO io{rows, 0, R, cnt, nullptr, nullptr, 0, 0, nullptr, family_counters(c_, FSIG_K_JOIN_COUNT)};
+      KTimer t(c_, FSIG_K_JOIN_COUNT);
+      if ((rc = launch_join<false>(c_, G_, L, S, io))) return rc;
+      t.done(0, 0);
+    }
+    if ((rc = scan_u32_to_u64(c_, cnt, off, R))) return rc;
+    uint64_t T;
+    if ((rc = read_u64(c_, off + R, &T, 1))) return rc;
+    if (T == 0) { c_->arena.release(mk); return FSIG_OK; }
+    if (last) T = std::min(T, remaining());
+    // Children per window: half of the free arena, each child costing its
+    // row plus the count/offset words it needs when expanded in turn.
+    const uint64_t per_child = 4ull * (p + 1) + 16;
+    uint64_t cap = c_->arena.avail() / 2 / per_child;
+    if (cap == 0) return set_err(c_, FSIG_ENOMEM, "join: arena exhausted for child rows");
+    uint64_t nwin = (T + cap - 1) / cap;
+    if (nwin > 2048) return set_err(c_, FSIG_ENOMEM, "join: level needs more than 2048 windows");
+    const uint64_t wcap = (T + nwin - 1) / nwin;  // balanced windows
+    std::vector<uint64_t> win(2 * nwin);
+    if (nwin == 1) {
+      win[0] = 0;
+      win[1] = R;
+    } else {
+      unsigned long long* dwin = (unsigned long long*)c_->arena.alloc(16 * nwin);
+      if (!dwin) return set_err(c_, FSIG_ENOMEM, "join: arena exhausted for windows");
+      k_window_rows<<<(uint32_t)((nwin + 255) / 256), 256, 0, c_->stream>>>(off, R, wcap, T, (uint32_t)nwin, dwin);
+      FSIG_HIP(c_, hipGetLastError());
+      if ((rc = read_u64(c_, dwin, win.data(), 2 * nwin))) return rc;
+    }
+    for (uint64_t w = 0; w < nwin && remaining() > 0; w++) {
+      const uint64_t lo = w * wcap, hi = std::min(T, lo + wcap);
+      if (lo >= hi) break;
+      const size_t m2 = c_->arena.mark();
+      uint32_t* child = (uint32_t*)c_->arena.alloc(4ull * (p + 1) * (hi - lo));
+      if (!child) return set_err(c_, FSIG_ENOMEM, "join: arena exhausted for a window");
+      JoinIO io{rows, win[2 * w], win[2 * w + 1], nullptr, off, child, lo, hi, nullptr,
+                family_counters(c_, FSIG_K_JOIN_WRITE)};
+      {
+        KTimer t(c_, FSIG_K_JOIN_WRITE);
+        if ((rc = launch_join<true>(c_, G_, L, S, io))) return rc;
+        t.done(0, 0);
+      }
+      if ((rc = expand(p + 1, child, hi - lo))) return rc;
+      c_->arena.release(m2);
+    }
+    c_->arena.release(mk);
+    return FSIG_OK;
+  }
+
+  fsig_ctx* c_;
+  Provider& pv_;
+  uint32_t k_, G_;
+  int mode_;
+  uint64_t max_emb_;
+  LeafSink* sink_;
+  uint64_t total_ = 0;
+};
+
+// Lanes per row: the tuned value, or the power of two at or above the
+// expected driver-list length, clamped to [4, 64].
+inline uint32_t pick_lanes(const fsig_ctx* c, double expected_list) {
+  if (c->tune.lanes_per_row) return c->tune.lanes_per_row;
+  uint32_t g = 4;
+  while (g < 64 && g < expected_list) g <<= 1;
+  return g;
+}

@@ -1,0 +1,181 @@
+// C ABI of libfsig_host.so (include/fsig_host.h).
+#include "../../include/fsig_host.h"
+
+#include <algorithm>
+#include <memory>
+
+#include "gen.h"
+#include "graph.h"
+#include "io.h"
+#include "transform.h"
+
+using namespace fsig;
+
+struct fsig_hgraph {
+  uint32_t n = 0;
+  int32_t num_labels = 1;
+  std::vector<uint32_t> off{0};
+  std::vector<uint32_t> adj;
+  std::vector<int32_t> labels;
+};
+
+struct fsig_hpcsr {
+  std::vector<PCSRUnit> units;
+};
+
+static void fix_num_labels(fsig_hgraph* g) {
+  int32_t mx = 0;
+  for (int32_t l : g->labels) mx = std::max(mx, l);
+  g->num_labels = mx + 1;
+}
+
+extern "C" {
+
+fsig_hgraph* fsig_hg_read(const char* path, int is_labeled, int* status) {
+  UndirectedPlainGraph plain;
+  int rc = read_plain_graph(path, is_labeled != 0, plain);
+  if (status) *status = rc;
+  if (rc) return nullptr;
+  Graph* out = nullptr;
+  UndirectedPlainToCSRTransformer t(true);
+  if (t.transform(&plain, out)) return nullptr;
+  std::unique_ptr<UndirectedCSRGraph> csr(static_cast<UndirectedCSRGraph*>(out));
+  auto* g = new fsig_hgraph();
+  g->n = csr->num_vertices_;
+  g->off = csr->offset_;
+  g->adj = csr->edges_;
+  g->labels = csr->labels_;
+  g->num_labels = std::max<int32_t>(csr->num_labels_, 1);
+  return g;
+}
+
+fsig_hgraph* fsig_hg_from_csr(uint32_t n, const uint32_t* off, const uint32_t* adj, const int32_t* labels) {
+  auto* g = new fsig_hgraph();
+  g->n = n;
+  g->off.assign(off, off + n + 1);
+  g->adj.assign(adj, adj + off[n]);
+  g->labels.assign(labels, labels + n);
+  fix_num_labels(g);
+  return g;
+}
+
+fsig_hgraph* fsig_hg_from_edges(uint32_t n, uint64_t m, const uint32_t* pairs, const int32_t* labels) {
+  EdgeList e(m);
+  for (uint64_t i = 0; i < m; i++) e[i] = {pairs[2 * i], pairs[2 * i + 1]};
+  for (auto& x : e)
+    if (x.first >= n || x.second >= n) return nullptr;
+  auto* g = new fsig_hgraph();
+  g->n = n;
+  csr_from_edge_list(n, e, g->off, g->adj);
+  if (labels) g->labels.assign(labels, labels + n); else g->labels.assign(n, 0);
+  fix_num_labels(g);
+  return g;
+}
+
+fsig_hgraph* fsig_hg_chung_lu(uint32_t n, uint64_t m, double gamma, uint64_t seed) {
+  EdgeList e = gen_chung_lu(n, m, gamma, seed);
+  auto* g = new fsig_hgraph();
+  g->n = n;
+  csr_from_edge_list(n, e, g->off, g->adj);
+  g->labels.assign(n, 0);
+  return g;
+}
+
+fsig_hgraph* fsig_hg_rmat(uint32_t scale, uint64_t m, double a, double b, double c, uint64_t seed) {
+  EdgeList e = gen_rmat(scale, m, a, b, c, seed);
+  auto* g = new fsig_hgraph();
+  g->n = 1u << scale;
+  csr_from_edge_list(g->n, e, g->off, g->adj);
+  g->labels.assign(g->n, 0);
+  return g;
+}
+
+int fsig_hg_set_labels(fsig_hgraph* g, int kind, int32_t num_labels, double s, uint64_t seed) {
+  if (!g || num_labels <= 0) return -1;
+  g->labels = kind == 0 ? labels_uniform(g->n, num_labels, seed) : labels_zipf(g->n, num_labels, s, seed);
+  g->num_labels = num_labels;
+  return 0;
+}
+
+int fsig_hg_write(const fsig_hgraph* g, const char* path) {
+  return write_graph(path, g->n, g->labels.data(), g->off.data(), g->adj.data());
+}
+
+void fsig_hg_info(const fsig_hgraph* g, uint32_t* n, uint64_t* nnz, int32_t* num_labels, uint32_t* max_degree) {
+  if (n) *n = g->n;
+  if (nnz) *nnz = g->off[g->n];
+  if (num_labels) *num_labels = g->num_labels;
+  if (max_degree) {
+    uint32_t md = 0;
+    for (uint32_t v = 0; v < g->n; v++) md = std::max(md, g->off[v + 1] - g->off[v]);
+    *max_degree = md;
+  }
+}
+
+void fsig_hg_export(const fsig_hgraph* g, uint32_t* off, uint32_t* adj, int32_t* labels) {
+  if (off) std::copy(g->off.begin(), g->off.end(), off);
+  if (adj) std::copy(g->adj.begin(), g->adj.end(), adj);
+  if (labels) std::copy(g->labels.begin(), g->labels.end(), labels);
+}
+
+const uint32_t* fsig_hg_off(const fsig_hgraph* g) { return g->off.data(); }
+const uint32_t* fsig_hg_adj(const fsig_hgraph* g) { return g->adj.data(); }
+const int32_t* fsig_hg_labels(const fsig_hgraph* g) { return g->labels.data(); }
+
+void fsig_hg_inverted_index(const fsig_hgraph* g, uint32_t* index, uint32_t* list) {
+  InvertedIndex<int32_t, uint32_t> inv(g->n, g->num_labels, g->labels.data());
+  std::copy(inv.index_.begin(), inv.index_.end(), index);
+  std::copy(inv.list_.begin(), inv.list_.end(), list);
+}
+
+void fsig_hg_signature(const fsig_hgraph* g, int column_oriented, int x86_shift, uint32_t* out) {
+  auto s = signature_from_csr(g->n, g->off.data(), g->adj.data(), g->labels.data(), column_oriented != 0,
+                              x86_shift != 0);
+  std::copy(s.begin(), s.end(), out);
+}
+
+void fsig_hg_free(fsig_hgraph* g) { delete g; }
+
+int fsig_hg_walk_query(const fsig_hgraph* g, uint32_t k, uint32_t min_e, uint32_t max_e, uint64_t seed, int bias,
+                       uint32_t max_attempts, int32_t* labels, uint32_t* edges, uint32_t* walk) {
+  WalkQuery q;
+  if (random_walk_query(g->n, g->off.data(), g->adj.data(), g->labels.data(), k, min_e, max_e, seed, bias,
+                        max_attempts, q))
+    return -1;
+  for (uint32_t i = 0; i < k; i++) {
+    labels[i] = q.labels[i];
+    walk[i] = q.data_vertices[i];
+  }
+  for (size_t i = 0; i < q.edges.size(); i++) {
+    edges[2 * i] = q.edges[i].first;
+    edges[2 * i + 1] = q.edges[i].second;
+  }
+  return (int)q.edges.size();
+}
+
+fsig_hpcsr* fsig_pcsr_build(const fsig_hgraph* g, int reference_unit_ids) {
+  auto* p = new fsig_hpcsr();
+  pcsr_from_csr(g->n, g->off.data(), g->adj.data(), g->labels.data(), g->num_labels, reference_unit_ids != 0,
+                p->units);
+  return p;
+}
+
+uint32_t fsig_pcsr_num_units(const fsig_hpcsr* p) { return (uint32_t)p->units.size(); }
+
+void fsig_pcsr_unit_info(const fsig_hpcsr* p, uint32_t u, uint32_t* keys, uint32_t* nedges) {
+  if (keys) *keys = p->units[u].num_keys_;
+  if (nedges) *nedges = p->units[u].num_edges_;
+}
+
+const uint32_t* fsig_pcsr_unit_offset(const fsig_hpcsr* p, uint32_t u) { return p->units[u].offset_.data(); }
+const uint32_t* fsig_pcsr_unit_edges(const fsig_hpcsr* p, uint32_t u) { return p->units[u].edges_.data(); }
+void fsig_pcsr_free(fsig_hpcsr* p) { delete p; }
+
+uint32_t fsig_murmur2(const void* key, int len, uint32_t seed) { return murmur_hash2(key, len, seed); }
+
+int fsig_write_embeddings(const char* path, uint32_t k, uint64_t count, const uint32_t* rows, uint64_t nrows) {
+  std::vector<uint32_t> r(rows, rows + nrows * k);
+  return write_embeddings(path, k, count, r);
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+// Survey-format reader/writer and the embedding file writer.
+#include "io.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+
+namespace fsig {
+
+const char* io_status_string(int s) {
+  switch (s) {
+    case kIoOk: return "ok";
+    case kIoCannotOpen: return "Error: cannot open file; undirected survey file";
+    case kIoBadHeader: return "Error: graph file format is not correct; undirected survey file";
+    case kIoBadRecord: return "Error: truncated or malformed record; undirected survey file";
+    case kIoMissingVertex: return "Error: a vertex id has no v line; undirected survey file";
+    case kIoEdgeOutOfRange: return "Error: an edge endpoint has no v line; undirected survey file";
+  }
+  return "Error: unknown";
+}
+
+namespace {
+
+struct Cursor {
+  const char* p;
+  const char* end;
+  void skip_ws() {
+    while (p < end && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r')) p++;
+  }
+  void skip_line() {
+    while (p < end && *p != '\n') p++;
+  }
+  bool next_char(char& c) {
+    skip_ws();
+    if (p >= end) return false;
+    c = *p++;
+    return true;
+  }
+  bool next_u64(uint64_t& v) {
+    skip_ws();
+    if (p >= end || *p < '0' || *p > '9') return false;
+    uint64_t x = 0;
+    while (p < end && *p >= '0' && *p <= '9') x = x * 10 + (uint64_t)(*p++ - '0');
+    v = x;
+    return true;
+  }
+  bool next_i64(int64_t& v) {
+    skip_ws();
+    bool neg = false;
+    if (p < end && *p == '-') { neg = true; p++; }
+    uint64_t u;
+    if (!next_u64(u)) return false;
+    v = neg ? -(int64_t)u : (int64_t)u;
+    return true;
+  }
+};
+
+bool slurp(const std::string& path, std::vector<char>& buf) {
+  FILE* f = fopen(path.c_str(), "rb");
+  if (!f) return false;
+  fseek(f, 0, SEEK_END);
+  long sz = ftell(f);
+  fseek(f, 0, SEEK_SET);
+  buf.resize(sz > 0 ? (size_t)sz : 0);
+  size_t got = sz > 0 ? fread(buf.data(), 1, (size_t)sz, f) : 0;
+  fclose(f);
+  return got == buf.size();
+}
+
+}  // namespace
+
+int read_plain_graph(const std::string& path, bool is_labeled, UndirectedPlainGraph& g) {
+  std::vector<char> buf;
+  if (!slurp(path, buf)) return kIoCannotOpen;
+  Cursor c{buf.data(), buf.data() + buf.size()};
+  char type;
+  uint64_t hn, hm;
+  if (!c.next_char(type) || type != 't') return kIoBadHeader;
+  if (!c.next_u64(hn) || !c.next_u64(hm)) return kIoBadHeader;
+  g = UndirectedPlainGraph();
+  g.is_labeled_ = is_labeled;
+  std::vector<std::pair<Vertex, Vertex>> edges;
+  edges.reserve((size_t)std::min<uint64_t>(hm, 1ull << 31));
+  while (c.next_char(type)) {
+    if (type == 'v') {
+      uint64_t id, deg;
+      int64_t label;
+      if (!c.next_u64(id) || !c.next_i64(label) || !c.next_u64(deg)) return kIoBadRecord;
+      if (id >= kInvalid || label < 0 || label > 0x7fffffff) return kIoBadRecord;
+      g.add_vertex((Vertex)id, is_labeled ? (Label)label : 0);
+    } else if (type == 'e') {
+      uint64_t a, b;
+      if (!c.next_u64(a) || !c.next_u64(b)) return kIoBadRecord;
+      if (a >= kInvalid || b >= kInvalid) return kIoBadRecord;
+      edges.emplace_back((Vertex)a, (Vertex)b);
+    } else {
+      c.skip_line();  // unknown record kind: ignored, as the reference does
+    }
+  }
+  for (Vertex v = 0; v < g.num_vertices_; v++)
+    if (g.labels_[v] == kInvalidLabel) return kIoMissingVertex;
+  for (auto& e : edges)
+    if (e.first >= g.num_vertices_ || e.second >= g.num_vertices_) return kIoEdgeOutOfRange;
+  // Pre-size adjacency from degrees so huge files do not re-grow vectors.
+  std::vector<uint32_t> deg(g.num_vertices_, 0);
+  for (auto& e : edges) { deg[e.first]++; deg[e.second]++; }
+  g.edges_.assign(g.num_vertices_, {});
+  for (Vertex v = 0; v < g.num_vertices_; v++) g.edges_[v].reserve(deg[v]);
+  for (auto& e : edges) g.add_edge(e.first, e.second);
+  g.finalize();
+  g.set_min_degree();
+  return kIoOk;
+}
+
+int write_graph(const std::string& path, Vertex n, const Label* labels, const Edge* off,
+                const Vertex* adj) {
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) return kIoCannotOpen;
+  uint64_t m = 0;
+  for (Vertex v = 0; v < n; v++)
+    for (Edge i = off[v]; i < off[v + 1]; i++)
+      if (adj[i] > v) m++;
+  std::vector<char> out;
+  out.reserve(1 << 20);
+  char line[96];
+  auto flush = [&](bool force) {
+    if (force || out.size() > (1 << 20)) {
+      fwrite(out.data(), 1, out.size(), f);
+      out.clear();
+    }
+  };
+  bool first = true;
+  auto emit = [&](int len) {
+    if (!first) out.push_back('\n');
+    first = false;
+    out.insert(out.end(), line, line + len);
+    flush(false);
+  };
+  emit(snprintf(line, sizeof line, "t %u %llu", n, (unsigned long long)m));
+  for (Vertex v = 0; v < n; v++)
+    emit(snprintf(line, sizeof line, "v %u %d %u", v, labels[v], off[v + 1] - off[v]));
+  for (Vertex v = 0; v < n; v++)
+    for (Edge i = off[v]; i < off[v + 1]; i++)
+      if (adj[i] > v) emit(snprintf(line, sizeof line, "e %u %u", v, adj[i]));
+  flush(true);
+  fclose(f);
+  return kIoOk;
+}
+
+int write_embeddings(const std::string& path, uint32_t k, uint64_t count, std::vector<uint32_t>& rows) {
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) return kIoCannotOpen;
+  size_t n = k ? rows.size() / k : 0;
+  std::vector<size_t> idx(n);
+  for (size_t i = 0; i < n; i++) idx[i] = i;
+  std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+    return std::lexicographical_compare(rows.begin() + a * k, rows.begin() + (a + 1) * k,
+                                        rows.begin() + b * k, rows.begin() + (b + 1) * k);
+  });
+  fprintf(f, "#embedding: %llu\n", (unsigned long long)count);
+  std::vector<uint32_t> sorted(rows.size());
+  for (size_t i = 0; i < n; i++) {
+    std::memcpy(&sorted[i * k], &rows[idx[i] * k], k * sizeof(uint32_t));
+    for (uint32_t j = 0; j < k; j++) fprintf(f, j + 1 < k ? "%u " : "%u\n", sorted[i * k + j]);
+  }
+  rows.swap(sorted);
+  fclose(f);
+  return kIoOk;
+}
+
+}  // namespace fsig

@@ -1,0 +1,256 @@
+// Storage transforms (CSR, PCSR, signatures) and MurmurHash2.
+#include "transform.h"
+
+#include <algorithm>
+#include <cstring>
+#include <deque>
+#include <thread>
+
+namespace fsig {
+
+uint32_t murmur_hash2(const void* key, int len, uint32_t seed) {
+  const uint32_t mul = 0x5bd1e995u;
+  const unsigned char* bytes = static_cast<const unsigned char*>(key);
+  uint32_t h = seed ^ (uint32_t)len;
+  int rest = len;
+  while (rest >= 4) {
+    uint32_t w;
+    std::memcpy(&w, bytes, 4);
+    w *= mul;
+    w ^= w >> 24;
+    w *= mul;
+    h = (h * mul) ^ w;
+    bytes += 4;
+    rest -= 4;
+  }
+  if (rest > 0) {
+    if (rest >= 3) h ^= (uint32_t)bytes[2] << 16;
+    if (rest >= 2) h ^= (uint32_t)bytes[1] << 8;
+    h ^= bytes[0];
+    h *= mul;
+  }
+  h ^= h >> 13;
+  h *= mul;
+  h ^= h >> 15;
+  return h;
+}
+
+static void sort_ranges(uint32_t n, const std::vector<uint32_t>& off, std::vector<uint32_t>& adj,
+                        int threads) {
+  if (threads <= 0) threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 100000) threads = 1;
+  auto work = [&](uint32_t lo, uint32_t hi) {
+    for (uint32_t v = lo; v < hi; v++) std::sort(adj.begin() + off[v], adj.begin() + off[v + 1]);
+  };
+  if (threads == 1) { work(0, n); return; }
+  std::vector<std::thread> pool;
+  uint32_t step = (n + threads - 1) / threads;
+  for (int t = 0; t < threads; t++) {
+    uint32_t lo = std::min<uint64_t>((uint64_t)t * step, n), hi = std::min<uint64_t>((uint64_t)(t + 1) * step, n);
+    pool.emplace_back(work, lo, hi);
+  }
+  for (auto& th : pool) th.join();
+}
+
+void csr_from_edge_list(uint32_t n, const std::vector<std::pair<uint32_t, uint32_t>>& edges,
+                        std::vector<uint32_t>& offset, std::vector<uint32_t>& adj, int threads) {
+  std::vector<uint32_t> deg((size_t)n + 1, 0);
+  for (auto& e : edges) {
+    if (e.first == e.second) continue;
+    deg[e.first]++;
+    deg[e.second]++;
+  }
+  offset.assign((size_t)n + 1, 0);
+  for (uint32_t v = 0; v < n; v++) offset[v + 1] = offset[v] + deg[v];
+  adj.resize(offset[n]);
+  std::vector<uint32_t> cur(offset.begin(), offset.end() - 1);
+  for (auto& e : edges) {
+    if (e.first == e.second) continue;
+    adj[cur[e.first]++] = e.second;
+    adj[cur[e.second]++] = e.first;
+  }
+  sort_ranges(n, offset, adj, threads);
+  // squeeze out duplicates
+  uint32_t w = 0;
+  uint32_t start = 0;
+  for (uint32_t v = 0; v < n; v++) {
+    uint32_t end = offset[v + 1];
+    uint32_t begin_w = w;
+    for (uint32_t i = start; i < end; i++)
+      if (w == begin_w || adj[w - 1] != adj[i]) adj[w++] = adj[i];
+    offset[v] = begin_w;
+    start = end;
+  }
+  offset[n] = w;
+  adj.resize(w);
+}
+
+int UndirectedPlainToCSRTransformer::transform(const Graph* input, Graph*& output) {
+  const auto* g = dynamic_cast<const UndirectedPlainGraph*>(input);
+  if (!g) return -1;
+  auto* csr = new UndirectedCSRGraph();
+  const Vertex n = g->num_vertices_;
+  csr->num_vertices_ = n;
+  csr->num_labels_ = g->num_labels_;
+  csr->is_labeled_ = g->is_labeled_;
+  csr->labels_.assign(g->labels_.begin(), g->labels_.begin() + n);
+  csr->offset_.assign((size_t)n + 1, 0);
+  std::vector<Vertex> scratch;
+  for (Vertex v = 0; v < n; v++) {
+    scratch.assign(g->edges_[v].begin(), g->edges_[v].end());
+    std::sort(scratch.begin(), scratch.end());
+    if (dedupe_) {
+      scratch.erase(std::unique(scratch.begin(), scratch.end()), scratch.end());
+      scratch.erase(std::remove(scratch.begin(), scratch.end(), v), scratch.end());
+    }
+    csr->offset_[v + 1] = csr->offset_[v] + (Edge)scratch.size();
+    csr->edges_.insert(csr->edges_.end(), scratch.begin(), scratch.end());
+  }
+  csr->num_edges_ = dedupe_ ? csr->offset_[n] / 2 : g->num_edges_;
+  Vertex md = n;
+  for (Vertex v = 0; v < n; v++) md = std::min(md, csr->degree(v));
+  csr->min_degree = md;
+  output = csr;
+  return 0;
+}
+
+void pcsr_from_csr(uint32_t n, const uint32_t* off, const uint32_t* adj, const int32_t* labels,
+                   int32_t num_labels, bool ref_ids, std::vector<PCSRUnit>& units) {
+  const uint32_t num_units = (uint32_t)((int64_t)num_labels * (num_labels + 1) / 2);
+  auto unit_of = [&](Label a, Label b) {
+    return (uint32_t)(ref_ids ? UndirectedPCSRGraph::reference_edge_label(a, b)
+                              : UndirectedPCSRGraph::injective_edge_label(a, b));
+  };
+  // Pass 1: per unit, keys in ascending vertex order and their list lengths.
+  std::vector<std::vector<uint32_t>> keys(num_units), key_len(num_units);
+  std::vector<uint32_t> last_key(num_units, kInvalid);
+  for (uint32_t v = 0; v < n; v++) {
+    for (uint32_t i = off[v]; i < off[v + 1]; i++) {
+      uint32_t u = unit_of(labels[v], labels[adj[i]]);
+      if (last_key[u] != v) {
+        last_key[u] = v;
+        keys[u].push_back(v);
+        key_len[u].push_back(0);
+      }
+      key_len[u].back()++;
+    }
+  }
+  units.assign(num_units, PCSRUnit());
+  std::vector<uint32_t> key_index;  // position of a key in keys[u]
+  for (uint32_t u = 0; u < num_units; u++) {
+    PCSRUnit& unit = units[u];
+    const uint32_t K = (uint32_t)keys[u].size();
+    unit.num_keys_ = K;
+    uint64_t ne = 0;
+    for (uint32_t l : key_len[u]) ne += l;
+    unit.num_edges_ = (uint32_t)ne;
+    unit.offset_.assign((size_t)32 * K, 0);
+    for (size_t w = 0; w < (size_t)32 * K; w += 2) unit.offset_[w] = kInvalid;
+    unit.edges_.assign(ne, kInvalid);
+    if (K == 0) continue;
+    // buckets by hash, keys in insertion order
+    std::vector<std::vector<uint32_t>> bucket(K);
+    for (uint32_t j = 0; j < K; j++) {
+      uint32_t key = keys[u][j];
+      bucket[murmur_hash2(&key, 4, kHashSeed) % K].push_back(j);
+    }
+    std::deque<uint32_t> empty;
+    for (uint32_t b = 0; b < K; b++)
+      if (bucket[b].empty()) empty.push_back(b);
+    for (uint32_t b = 0; b < K; b++) {
+      const auto& bk = bucket[b];
+      if (bk.empty()) continue;
+      uint32_t group = b;
+      size_t placed = 0;
+      const size_t full = bk.size() / 15;
+      for (size_t f = 0; f < full; f++) {
+        for (uint32_t s = 0; s < 15; s++) unit.offset_[(size_t)32 * group + 2 * s] = keys[u][bk[placed++]];
+        uint32_t next = empty.front();
+        empty.pop_front();
+        unit.offset_[(size_t)32 * group + 30] = next;
+        group = next;
+      }
+      for (uint32_t s = 0; placed < bk.size(); s++)
+        unit.offset_[(size_t)32 * group + 2 * s] = keys[u][bk[placed++]];
+    }
+    // key -> index map for list lookup (replaces the linear find)
+    key_index.assign(0, 0);
+    // Lists are written group by group, slot by slot.
+    uint32_t pos = 0;
+    for (uint32_t grp = 0; grp < K; grp++) {
+      for (uint32_t s = 0; s < 15; s++) {
+        uint32_t key = unit.offset_[(size_t)32 * grp + 2 * s];
+        if (key == kInvalid) break;
+        unit.offset_[(size_t)32 * grp + 2 * s + 1] = pos;
+        for (uint32_t i = off[key]; i < off[key + 1]; i++)
+          if (unit_of(labels[key], labels[adj[i]]) == u) unit.edges_[pos++] = adj[i];
+        unit.offset_[(size_t)32 * grp + 2 * (s + 1) + 1] = pos;
+      }
+    }
+  }
+}
+
+int UndirectedPlainToPCSRTransformer::transform(const Graph* input, Graph*& output) {
+  const auto* g = dynamic_cast<const UndirectedPlainGraph*>(input);
+  if (!g) return -1;
+  // Same adjacency the reference uses: each list sorted ascending
+  // (graph_format_transformer.cc:79-83), duplicates kept.
+  const Vertex n = g->num_vertices_;
+  std::vector<uint32_t> off((size_t)n + 1, 0), adj;
+  for (Vertex v = 0; v < n; v++) off[v + 1] = off[v] + (uint32_t)g->edges_[v].size();
+  adj.resize(off[n]);
+  for (Vertex v = 0; v < n; v++) {
+    std::copy(g->edges_[v].begin(), g->edges_[v].end(), adj.begin() + off[v]);
+    std::sort(adj.begin() + off[v], adj.begin() + off[v + 1]);
+  }
+  auto* p = new UndirectedPCSRGraph();
+  p->num_vertices_ = n;
+  p->num_edges_ = g->num_edges_;
+  p->num_labels_ = g->num_labels_;
+  p->is_labeled_ = g->is_labeled_;
+  p->labels_.assign(g->labels_.begin(), g->labels_.begin() + n);
+  p->reference_unit_ids_ = reference_unit_ids_;
+  pcsr_from_csr(n, off.data(), adj.data(), p->labels_.data(), p->num_labels_, reference_unit_ids_, p->units_);
+  p->num_units_ = (Vertex)p->units_.size();
+  output = p;
+  return 0;
+}
+
+std::vector<uint32_t> signature_from_csr(uint32_t n, const uint32_t* off, const uint32_t* adj,
+                                         const int32_t* labels, bool column_oriented, bool x86_shift) {
+  const uint32_t words = kSigWords;                // 16
+  const uint32_t slots = (kSigBits - 32) / 2;      // 240 two-bit counters
+  std::vector<uint32_t> row((size_t)words * n, 0);
+  for (uint32_t v = 0; v < n; v++) {
+    uint32_t* sig = &row[(size_t)v * words];
+    sig[0] = (uint32_t)labels[v];
+    for (uint32_t i = off[v]; i < off[v + 1]; i++) {
+      int32_t nl = labels[adj[i]];
+      uint32_t h = murmur_hash2(&nl, 4, kHashSeed) % slots;
+      uint32_t word = 1 + (2 * h) / 32;
+      uint32_t bit = (2 * h) % 32;
+      uint32_t shift = x86_shift ? ((2 * bit) & 31u) : bit;
+      uint32_t cnt = (sig[word] >> shift) & 3u;
+      uint32_t next = cnt == 0 ? 1u : 3u;
+      sig[word] |= next << shift;
+    }
+  }
+  if (!column_oriented) return row;
+  std::vector<uint32_t> col((size_t)words * n);
+  for (uint32_t w = 0; w < words; w++)
+    for (uint32_t v = 0; v < n; v++) col[(size_t)w * n + v] = row[(size_t)v * words + w];
+  return col;
+}
+
+std::vector<uint32_t> UndirectedPlainToPCSRTransformer::build_signature(bool column_oriented,
+                                                                        const UndirectedPlainGraph& g,
+                                                                        bool x86_shift) {
+  const Vertex n = g.num_vertices_;
+  std::vector<uint32_t> off((size_t)n + 1, 0), adj;
+  for (Vertex v = 0; v < n; v++) off[v + 1] = off[v] + (uint32_t)g.edges_[v].size();
+  adj.resize(off[n]);
+  for (Vertex v = 0; v < n; v++) std::copy(g.edges_[v].begin(), g.edges_[v].end(), adj.begin() + off[v]);
+  return signature_from_csr(n, off.data(), adj.data(), g.labels_.data(), column_oriented, x86_shift);
+}
+
+}  // namespace fsig

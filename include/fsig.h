@@ -1,0 +1,142 @@
+/*
+ * fsig.h - C ABI of libfsig_hip.so, the MI355X (gfx950) subgraph-matching
+ * engine that replaces the GPU side of FSIG's matchers.
+ *
+ * Plain pointers and sizes only.  Every call returns 0 on success or a
+ * negative FSIG_E* code; the message of the last failure is kept per context
+ * (fsig_last_error).  Nothing in this library calls exit().
+ *
+ * Semantics of a match (the output contract of the reference's MMatch / GSI):
+ * an embedding is a map phi from query vertices to data vertices that is
+ * injective, keeps labels (all labels are 0 for an unlabeled graph) and maps
+ * every query edge to a data edge.  Non-edges of the query are unconstrained
+ * and automorphic images are distinct embeddings.  Rows handed back are k
+ * data-vertex ids in query-vertex-id order.
+ */
+#ifndef FSIG_H
+#define FSIG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fsig_ctx fsig_ctx;
+typedef struct fsig_graph fsig_graph;
+typedef struct fsig_server fsig_server;
+
+enum {
+  FSIG_OK = 0,
+  FSIG_EINVAL = -1,       /* bad argument, malformed graph or query        */
+  FSIG_ENOMEM = -2,       /* request exceeds the HBM budget of the context */
+  FSIG_EHIP = -3,         /* HIP runtime error                             */
+  FSIG_EUNSUPPORTED = -4, /* e.g. more than 32 query vertices              */
+  FSIG_ENOTFOUND = -5,    /* unknown ticket / missing PCSR                 */
+  FSIG_EAGAIN = -6        /* server: not finished yet / queue full         */
+};
+
+enum { FSIG_ALG_MMATCH = 1, FSIG_ALG_GSI = 2 };
+enum { FSIG_MODE_COUNT = 0, FSIG_MODE_ENUMERATE = 1 };
+
+#define FSIG_MAX_QUERY_VERTICES 32
+
+/* Opens device `device`, creates the context's HIP stream and reserves an
+ * arena of `hbm_budget_bytes` of device memory for per-query state
+ * (candidate sets, candidate-edge lists, intermediate embedding tables).
+ * 0 selects 3/4 of the device memory free at the call.
+ * Replaces GPUSubgraphMatcher::init_GPU (match/subgraph_match.h:51-80) and
+ * the MemoryManager budget of totalGlobalMem (match/MMatch.h:158-160,
+ * memory_manager/memory_manager.h:16-28). */
+int fsig_ctx_create(int device, uint64_t hbm_budget_bytes, fsig_ctx** out);
+void fsig_ctx_destroy(fsig_ctx* ctx);
+const char* fsig_last_error(const fsig_ctx* ctx);
+
+/* Uploads a data graph in CSR form; it stays resident until fsig_graph_free.
+ * off[n+1], adj[off[n]] strictly ascending per vertex (no duplicates),
+ * labels[n] in [0, num_labels), inv_index[num_labels+1] / inv_list[n] the
+ * label -> vertex inverted index (vertices ascending within a label).
+ * Replaces MMatch::copy_graph_to_GPU (match/MMatch.cu:14-32); the inverted
+ * index is the one main() builds (main/run.cc:122,130). */
+int fsig_graph_upload_csr(fsig_ctx* ctx, uint32_t n, const uint32_t* off, const uint32_t* adj,
+                          const int32_t* labels, uint32_t num_labels, const uint32_t* inv_index,
+                          const uint32_t* inv_list, fsig_graph** out);
+
+/* Adds GSI's storage to an uploaded graph: num_units PCSR units (one per
+ * unordered label pair; unit u has unit_keys[u] groups of 32 words in
+ * unit_offset[u] and unit_nedges[u] neighbour ids in unit_edges[u], the
+ * layout of graph/graph.h:90-95) and the column-major [16][n] signature
+ * table of graph_format_transformer.cc:217-281.  reference_unit_ids != 0
+ * says the units are numbered as graph/graph.h:107-122 numbers them.
+ * Replaces the per-join unit uploads of GSI::join_embeddings
+ * (match/GSI.cu:234-243) and the signature upload of GSI::signature_filter
+ * (match/GSI.cu:38-45): everything is uploaded once and stays resident. */
+int fsig_graph_upload_pcsr(fsig_ctx* ctx, fsig_graph* g, uint32_t num_units,
+                           const uint32_t* unit_keys, const uint32_t* const* unit_offset,
+                           const uint32_t* unit_nedges, const uint32_t* const* unit_edges,
+                           const uint32_t* sig_colmajor, int reference_unit_ids);
+void fsig_graph_free(fsig_ctx* ctx, fsig_graph* g);
+
+/* Matches one query (k <= 32 vertices; qoff[k+1] / qadj CSR, each
+ * undirected edge stored in both directions; qsig[k*16] row-major query
+ * signatures, used by alg 2 only, may be NULL for alg 1).
+ *   alg  FSIG_ALG_MMATCH: label/degree filter over the inverted index,
+ *        count-based refinement, candidate-edge CSR, memory-blocked join.
+ *        Replaces MMatch::match (match/MMatch.cu:2077-2122).
+ *        FSIG_ALG_GSI: signature filter, join over the resident PCSR.
+ *        Replaces GSI::match (match/GSI.cu:329-358).
+ *   mode FSIG_MODE_COUNT: *count_out = number of embeddings (u64).
+ *        FSIG_MODE_ENUMERATE: embeddings are materialised in HBM; at most
+ *        max_embeddings of them (0 = no bound), *count_out = number
+ *        enumerated; the first min(count, emb_cap) rows are copied to
+ *        emb_out (host, k words per row, query-vertex order) when
+ *        emb_out != NULL.  Which rows form a bounded prefix is unspecified.
+ */
+int fsig_match(fsig_ctx* ctx, const fsig_graph* g, uint32_t k, const int32_t* qlabels,
+               const uint32_t* qoff, const uint32_t* qadj, const uint32_t* qsig, int alg, int mode,
+               uint64_t max_embeddings, uint64_t* count_out, uint32_t* emb_out, uint64_t emb_cap);
+
+/* Per-kernel-family statistics of the context, accumulated since the last
+ * fsig_stats_reset.  With profiling on, every launch of the family is
+ * bracketed by HIP events on the context stream and its time is summed. */
+enum {
+  FSIG_K_FILTER = 0,   /* label/degree or signature filter + compaction  */
+  FSIG_K_REFINE = 1,   /* count-based refinement passes                  */
+  FSIG_K_CEDGE = 2,    /* candidate-edge CSR count + fill                */
+  FSIG_K_JOIN_COUNT = 3,
+  FSIG_K_JOIN_WRITE = 4,
+  FSIG_K_SCAN = 5,
+  FSIG_K_BATCH = 6,    /* persistent batched join                        */
+  FSIG_K_NUM = 7
+};
+int fsig_set_profiling(fsig_ctx* ctx, int on);
+int fsig_stats_reset(fsig_ctx* ctx);
+/* launches, summed device ms (profiling on), algorithmic bytes (every
+ * logical element a kernel reads or writes counted once per use, at its
+ * natural width) and units processed (rows, candidates or children). */
+int fsig_kernel_stats(const fsig_ctx* ctx, int family, uint64_t* launches, double* ms,
+                      uint64_t* alg_bytes, uint64_t* units);
+/* Candidate-set sizes of the last fsig_match, per query vertex (k entries). */
+int fsig_last_candidates(const fsig_ctx* ctx, uint32_t k, uint32_t* sizes);
+/* Peak arena bytes of the last fsig_match. */
+uint64_t fsig_last_peak_bytes(const fsig_ctx* ctx);
+
+/* Join tuning (alg 1 and 2), all fields set at once; 0 selects the automatic
+ * choice.  lanes_per_row in {4,8,16,32,64}: lanes of a wavefront that share
+ * one partial embedding (auto: the power of two at or above the mean list
+ * length).  strategy: 0 auto, 1 binary search of the other lists in global
+ * memory, 2 stage the other lists in LDS when they fit (lds_tile words per
+ * row).  refine_passes: cap on refinement passes (0 = 4). */
+typedef struct fsig_tune {
+  uint32_t lanes_per_row;
+  uint32_t strategy;
+  uint32_t lds_tile;
+  uint32_t refine_passes; /* cap on refinement passes (default 4) */
+} fsig_tune;
+int fsig_set_tune(fsig_ctx* ctx, const fsig_tune* t);
+int fsig_get_tune(const fsig_ctx* ctx, fsig_tune* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FSIG_H */

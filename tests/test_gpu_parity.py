@@ -1,0 +1,195 @@
+"""Parity of the HIP engine (through the C ABI) with the oracle and the golden
+fixtures.  Bit-exact: embedding counts (u64) and the sorted embedding sets."""
+import numpy as np
+import pytest
+
+import fsig_amd as F
+import oracle as O
+from conftest import adjm_from_edges, csr_from_edges, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _sorted_rows(a):
+    a = np.asarray(a, dtype=np.uint32)
+    if a.size == 0:
+        return a.reshape(0, a.shape[1] if a.ndim == 2 else 0)
+    return a[np.lexsort(a.T[::-1])]
+
+
+def _check_set(dg, q, alg, off, adj, lab):
+    want_cnt, want_rows = O.match(q.labels, q.adjm(), off, adj, lab, threads=8, cap=10 ** 7)
+    r = F.match(dg, q, alg, F.MODE_ENUMERATE, emb_cap=max(want_cnt, 1))
+    assert r.count == want_cnt
+    assert np.array_equal(_sorted_rows(r.embeddings), _sorted_rows(want_rows))
+    rc = F.match(dg, q, alg, F.MODE_COUNT)
+    assert rc.count == want_cnt
+    return want_cnt
+
+
+def test_vf2_fixtures_both_algorithms(gpu_ctx):
+    cache = {}
+    for c in load_golden("vf2_cases.json"):
+        key = c["graph"]
+        if key not in cache:
+            g = F.HostGraph.from_edges(c["n"], c["edges"], c["labels"])
+            cache[key] = gpu_ctx.upload(g, pcsr=True)
+        dg = cache[key]
+        q = F.Query(c["k"], c["qlabels"], c["qedges"])
+        for alg in (F.ALG_MMATCH, F.ALG_GSI):
+            r = F.match(dg, q, alg, F.MODE_ENUMERATE, emb_cap=max(c["count"], 1))
+            assert r.count == c["count"], (c["graph"], c["query"], alg)
+            assert _sorted_rows(r.embeddings).tolist() == c["rows"], (c["graph"], c["query"], alg)
+
+
+def test_c1_diamond_full_enumeration(gpu_ctx):
+    """10,000-vertex Chung-Lu (gamma 2.5, 50,000 edges, 8 uniform labels), the
+    diamond query from a seeded random walk: full set, alg 1 and alg 2."""
+    g = F.HostGraph.chung_lu(10000, 50000, 2.5, 1).set_labels("uniform", 8, seed=2)
+    off, adj, lab = g.arrays()
+    dg = gpu_ctx.upload(g, pcsr=True)
+    q = g.walk_query(4, 5, 5, seed=3)
+    for alg in (F.ALG_MMATCH, F.ALG_GSI):
+        assert _check_set(dg, q, alg, off, adj, lab) >= 1
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_walk_queries_labeled(gpu_ctx, seed):
+    g = F.HostGraph.chung_lu(20000, 120000, 2.3, 10 + seed).set_labels("zipf", 6, 1.2, 20 + seed)
+    off, adj, lab = g.arrays()
+    dg = gpu_ctx.upload(g, pcsr=True)
+    for k in (3, 5, 6, 8):
+        for dense in (False, True):
+            lo, hi = ((3 * k + 1) // 2, 99) if dense else (k - 1, (3 * k - 1) // 2)
+            try:
+                q = g.walk_query(k, lo, hi, seed=1000 * seed + 10 * k + dense, bias=2 if dense else 0,
+                                 max_attempts=20000)
+            except F.FsigError:
+                continue
+            cnt, _ = O.match(q.labels, q.adjm(), off, adj, lab, threads=8)
+            if cnt > 2_000_000:
+                continue
+            for alg in (F.ALG_MMATCH, F.ALG_GSI):
+                _check_set(dg, q, alg, off, adj, lab)
+
+
+def test_unlabeled_patterns_count(gpu_ctx):
+    g = F.HostGraph.rmat(12, 40000, seed=7)
+    off, adj, _ = g.arrays()
+    lab = np.zeros(g.n, np.int32)
+    dg = gpu_ctx.upload(g, pcsr=True)
+    pats = {"triangle": (3, [(0, 1), (1, 2), (0, 2)]), "cycle4": (4, [(0, 1), (1, 2), (2, 3), (3, 0)]),
+            "clique4": (4, [(a, b) for a in range(4) for b in range(a + 1, 4)]),
+            "house": (5, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0), (1, 4)])}
+    for name, (k, e) in pats.items():
+        q = F.Query(k, np.zeros(k, np.int32), e)
+        want, _ = O.match(q.labels, q.adjm(), off, adj, lab, threads=8)
+        for alg in (F.ALG_MMATCH, F.ALG_GSI):
+            assert F.match(dg, q, alg, F.MODE_COUNT).count == want, (name, alg)
+
+
+def test_candidate_sets_equal_oracle_refinement(gpu_ctx):
+    g = F.HostGraph.chung_lu(8000, 40000, 2.5, 31).set_labels("uniform", 4, seed=32)
+    off, adj, lab = g.arrays()
+    dg = gpu_ctx.upload(g)
+    passes = gpu_ctx.get_tune()["refine_passes"]
+    for s in range(5):
+        q = g.walk_query(6, 5, 12, seed=500 + s)
+        F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT)
+        got = gpu_ctx.last_candidates(q.k)
+        want = O.refine(q.labels, q.adjm(), off, adj, lab, passes)
+        assert np.array_equal(got, want)
+
+
+def test_gsi_candidate_sets_equal_oracle_signature_filter(gpu_ctx):
+    g = F.HostGraph.chung_lu(8000, 40000, 2.5, 41).set_labels("zipf", 5, 1.2, 42)
+    dg = gpu_ctx.upload(g, pcsr=True)
+    dsig = g.signature(True)
+    for s in range(4):
+        q = g.walk_query(5, 4, 10, seed=700 + s)
+        F.match(dg, q, F.ALG_GSI, F.MODE_COUNT)
+        got = gpu_ctx.last_candidates(q.k)
+        qsig = q.signature().reshape(q.k, 16)
+        want = [len(O.sig_filter(dsig, qsig[u])) for u in range(q.k)]
+        assert got.tolist() == want
+
+
+def test_edge_cases(gpu_ctx):
+    g = F.HostGraph.chung_lu(3000, 12000, 2.5, 51).set_labels("uniform", 3, seed=52)
+    off, adj, lab = g.arrays()
+    dg = gpu_ctx.upload(g, pcsr=True)
+    # single vertex: every vertex of the label
+    q1 = F.Query(1, [1], np.zeros((0, 2)))
+    for alg in (F.ALG_MMATCH, F.ALG_GSI):
+        assert F.match(dg, q1, alg).count == int((lab == 1).sum())
+    # label absent from the data graph
+    qx = F.Query(2, [1, 7], [(0, 1)])
+    assert F.match(dg, qx, F.ALG_MMATCH).count == 0
+    # disconnected query (two edges)
+    q2 = F.Query(4, [0, 1, 0, 1], [(0, 1), (2, 3)])
+    for alg in (F.ALG_MMATCH, F.ALG_GSI):
+        _check_set(dg, q2, alg, off, adj, lab)
+    # a clique no vertex can host: empty candidate sets, count 0
+    k = 12
+    qk = F.Query(k, [0] * k, [(a, b) for a in range(k) for b in range(a + 1, k)])
+    assert F.match(dg, qk, F.ALG_MMATCH).count == O.match(qk.labels, qk.adjm(), off, adj, lab, threads=8)[0]
+    # invalid query: self-loop, out-of-range vertex
+    with pytest.raises(F.FsigError):
+        F.match(dg, F.Query(2, [0, 0], [(0, 0)]), F.ALG_MMATCH)
+
+
+def test_bounded_enumeration(gpu_ctx):
+    g = F.HostGraph.chung_lu(5000, 30000, 2.3, 61).set_labels("uniform", 2, seed=62)
+    off, adj, lab = g.arrays()
+    dg = gpu_ctx.upload(g)
+    q = F.Query(3, [0, 1, 0], [(0, 1), (1, 2)])
+    total = F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT).count
+    bound = total // 3
+    r = F.match(dg, q, F.ALG_MMATCH, F.MODE_ENUMERATE, max_embeddings=bound, emb_cap=bound)
+    assert r.count == bound and len(r.embeddings) == bound
+    rows = r.embeddings
+    assert len({tuple(x) for x in rows.tolist()}) == bound  # distinct
+    nb = [set(adj[off[v]:off[v + 1]].tolist()) for v in range(g.n)]
+    for a, b, c in rows[:2000].tolist():  # each row is an embedding
+        assert lab[a] == 0 and lab[b] == 1 and lab[c] == 0 and a != c and b in nb[a] and c in nb[b]
+
+
+def test_small_arena_forces_windows():
+    """A 24 MiB arena splits levels into many windows; the set must not change."""
+    g = F.HostGraph.chung_lu(6000, 40000, 2.3, 71).set_labels("uniform", 2, seed=72)
+    off, adj, lab = g.arrays()
+    ctx = F.Context(0, 24 << 20)
+    try:
+        dg = ctx.upload(g)
+        q = F.Query(4, [0, 1, 0, 1], [(0, 1), (1, 2), (2, 3)])
+        want, _ = O.match(q.labels, q.adjm(), off, adj, lab, threads=8)
+        assert F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT).count == want
+        big = F.Context(0, 1 << 30)
+        try:
+            bdg = big.upload(g)
+            r_big = F.match(bdg, q, F.ALG_MMATCH, F.MODE_ENUMERATE, emb_cap=want)
+        finally:
+            pass
+        r = F.match(dg, q, F.ALG_MMATCH, F.MODE_ENUMERATE, emb_cap=want)
+        assert r.count == want
+        assert np.array_equal(_sorted_rows(r.embeddings), _sorted_rows(r_big.embeddings))
+        del bdg
+        big.close()
+    finally:
+        del dg
+        ctx.close()
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
+def test_lanes_per_row_do_not_change_results(gpu_ctx, lanes):
+    g = F.HostGraph.chung_lu(8000, 50000, 2.3, 81).set_labels("uniform", 3, seed=82)
+    off, adj, lab = g.arrays()
+    dg = gpu_ctx.upload(g, pcsr=True)
+    q = g.walk_query(5, 5, 9, seed=83)
+    want, _ = O.match(q.labels, q.adjm(), off, adj, lab, threads=8)
+    try:
+        gpu_ctx.set_tune(lanes_per_row=lanes)
+        for alg in (F.ALG_MMATCH, F.ALG_GSI):
+            assert F.match(dg, q, alg, F.MODE_COUNT).count == want
+    finally:
+        gpu_ctx.set_tune()

@@ -1,0 +1,157 @@
+"""Host side of the drop-in: reader, CSR, inverted index, PCSR, signatures,
+MurmurHash2 and the generators, against the reference's own host code
+(oracle/_ref/ref_dump, compiled from the reference sources) and against the
+committed fixtures it produced (tests/golden/ref_tiny.json)."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import fsig_amd as F
+import oracle as O
+from conftest import load_golden
+
+
+def test_ref_fixtures_csr_inverted_signature_murmur():
+    for case in load_golden("ref_tiny.json"):
+        g = F.HostGraph.from_edges(case["n"], case["edges"], case["labels"])
+        off, adj, lab = g.arrays()
+        assert off.tolist() == case["OFF"]
+        assert adj.tolist() == case["ADJ"]
+        idx, lst = g.inverted_index()
+        assert idx.tolist() == case["INVIDX"] and lst.tolist() == case["INVLIST"]
+        assert g.signature(False).tolist() == case["SIGQ"]
+        assert g.signature(True).tolist() == case["SIGD"]
+        # the oracle's restatement agrees with the reference too
+        assert O.signature(off, adj, lab, True).tolist() == case["SIGD"]
+        hashes = [F.host_lib().fsig_murmur2(np.array([i], np.uint32).ctypes.data, 4, 17) for i in range(256)]
+        assert hashes == case["MURMUR"]
+        assert [O.murmur2_u32(i) for i in range(256)] == case["MURMUR"]
+
+
+def test_ref_fixtures_pcsr_layout():
+    for case in load_golden("ref_tiny.json"):
+        g = F.HostGraph.from_edges(case["n"], case["edges"], case["labels"])
+        p = g.pcsr(reference_unit_ids=True)
+        assert p.num_units == len(case["units"])
+        for u, ref in enumerate(case["units"]):
+            o, e = p.unit(u)
+            assert o.tolist() == ref["offset"]
+            assert e.tolist() == ref["edges"]
+
+
+def _pcsr_lookup(o, e, key):
+    """Bucket walk of the PCSR layout (graph/graph.h:90-95)."""
+    K = len(o) // 32
+    if K == 0:
+        return None
+    grp = O.murmur2_u32(key) % K
+    for _ in range(K + 1):
+        words = o[32 * grp: 32 * grp + 32]
+        for s in range(15):
+            if words[2 * s] == 0xFFFFFFFF:
+                return None
+            if words[2 * s] == key:
+                return e[words[2 * s + 1]: words[2 * s + 3]]
+        grp = words[30]
+        if grp == 0xFFFFFFFF:
+            return None
+    return None
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_pcsr_injective_units_hold_label_partitioned_lists(seed):
+    g = F.HostGraph.chung_lu(3000, 15000, 2.3, seed).set_labels("zipf", 6, 1.2, seed)
+    off, adj, lab = g.arrays()
+    p = g.pcsr(reference_unit_ids=False)
+    L = g.num_labels
+    rng = np.random.default_rng(seed)
+    for v in rng.choice(g.n, 200, replace=False):
+        for l2 in range(L):
+            hi, lo = max(lab[v], l2), min(lab[v], l2)
+            o, e = p.unit(hi * (hi + 1) // 2 + lo)
+            want = [int(x) for x in adj[off[v]:off[v + 1]] if lab[x] == l2]
+            got = _pcsr_lookup(o, e, int(v))
+            assert (list(got) if got is not None else []) == want
+
+
+def test_generated_graph_matches_reference_on_file():
+    if not O.ref_available():
+        pytest.skip("oracle/_ref not built")
+    g = F.HostGraph.chung_lu(5000, 20000, 2.5, 11).set_labels("zipf", 8, 1.2, 12)
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "g.graph")
+        g.write(path)
+        r = O.ref_dump(path, True, 16)
+        g2 = F.HostGraph.read(path, True)
+    off, adj, lab = g.arrays()
+    off2, adj2, lab2 = g2.arrays()
+    assert np.array_equal(off, off2) and np.array_equal(adj, adj2) and np.array_equal(lab, lab2)
+    assert np.array_equal(r["OFF"], off) and np.array_equal(r["ADJ"], adj)
+    assert np.array_equal(r["SIGD"], g.signature(True))
+    p = g.pcsr(reference_unit_ids=True)
+    for u in range(p.num_units):
+        o, e = p.unit(u)
+        assert np.array_equal(o, r["units"][u]["offset"]) and np.array_equal(e, r["units"][u]["edges"])
+
+
+def _write(td, text):
+    path = os.path.join(td, "x.graph")
+    with open(path, "w") as f:
+        f.write(text)
+    return path
+
+
+def test_reader_trailing_newline_adds_no_phantom_edge():
+    with tempfile.TemporaryDirectory() as td:
+        p = _write(td, "t 3 2\nv 0 1 1\nv 1 2 2\nv 2 1 1\ne 0 1\ne 1 2\n\n")
+        g = F.HostGraph.read(p, True)
+        assert g.nnz == 4 and g.n == 3
+        assert g.arrays()[2].tolist() == [1, 2, 1]
+        g0 = F.HostGraph.read(p, False)
+        assert g0.arrays()[2].tolist() == [0, 0, 0]
+
+
+def test_reader_errors():
+    with tempfile.TemporaryDirectory() as td:
+        with pytest.raises(F.FsigError) as e:
+            F.HostGraph.read(os.path.join(td, "missing.graph"))
+        assert e.value.code == -1
+        with pytest.raises(F.FsigError) as e:
+            F.HostGraph.read(_write(td, "x 1 2\n"))
+        assert e.value.code == -2
+        with pytest.raises(F.FsigError) as e:  # id 1 has no v line
+            F.HostGraph.read(_write(td, "t 3 1\nv 0 0 1\nv 2 0 1\ne 0 2"))
+        assert e.value.code == -4
+        with pytest.raises(F.FsigError) as e:  # endpoint without v line
+            F.HostGraph.read(_write(td, "t 2 1\nv 0 0 1\nv 1 0 1\ne 0 5"))
+        assert e.value.code == -5
+
+
+def test_generators_are_deterministic_and_sized():
+    a = F.HostGraph.chung_lu(2000, 9000, 2.3, 5)
+    b = F.HostGraph.chung_lu(2000, 9000, 2.3, 5)
+    assert a.nnz == 2 * 9000 and np.array_equal(a.arrays()[1], b.arrays()[1])
+    off, adj, _ = a.arrays()
+    assert off[-1] - off[-2] > 0  # max id is not isolated
+    r = F.HostGraph.rmat(12, 20000, seed=3)
+    assert r.n == 4096 and r.nnz == 40000
+    roff = r.arrays()[0]
+    assert roff[-1] - roff[-2] > 0
+    z = a.set_labels("zipf", 16, 1.2, 9).arrays()[2]
+    freq = np.bincount(z, minlength=16)
+    assert freq[0] > freq[1] > freq[3] > freq[15]
+
+
+def test_walk_query_is_induced_and_has_a_witness():
+    g = F.HostGraph.chung_lu(10000, 50000, 2.5, 1).set_labels("uniform", 8, seed=2)
+    off, adj, lab = g.arrays()
+    q = g.walk_query(4, 5, 5, seed=3)
+    assert q.num_edges == 5  # 4 vertices + 5 edges = the diamond
+    w = q.walk
+    for i in range(4):
+        assert lab[w[i]] == q.labels[i]
+        for j in range(i + 1, 4):
+            has = w[j] in set(adj[off[w[i]]:off[w[i] + 1]].tolist())
+            assert has == ((i, j) in {tuple(e) for e in q.edges.tolist()})
