@@ -1,0 +1,181 @@
+"""Benchmark of the MI355X subgraph-matching engine.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
+
+A step is one pass of the hot path over the C2 workload (the dblp-shaped
+configuration the headline metric is quoted on): a Chung-Lu data graph of
+317,080 vertices / 1,049,866 edges with 16 Zipf(1.2) labels, resident in HBM,
+and its two 8-vertex random-walk queries (dense and sparse), each matched by
+alg 1 (fsig_match, FSIG_MODE_COUNT: every embedding is enumerated level by
+level; the last level is counted, not written).  Inputs are resident before
+the timed region; every fsig_match returns only after its stream drained.
+
+Prints one JSON line.  `value` is embeddings/s of the timed steps; `roofline`
+describes the dominant kernel family measured with HIP events on the
+engine's stream during the timed region; `cpu_baseline` is the oracle
+(oracle/oracle.c, a restatement, "port") on the host cores with a bounded
+sample of the same queries.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "embeddings/s & queries/s (batched vs per-query), p50/p99 latency, % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(g, queries, seconds_target=20.0):
+    """Oracle on the host: the dense query is counted in full (it also checks
+    the GPU count at full size); the sparse query is sampled: the search
+    stops after a bounded number of embeddings, sized from a short probe to
+    take about the rest of the time budget."""
+    import oracle as O
+    off, adj, lab = g.arrays()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    total_emb, total_s, parts, checks = 0, 0.0, [], {}
+    for name, q in queries:
+        if name == "dense":
+            t = time.perf_counter()
+            cnt, _, stopped = O.match_partial(q.labels, q.adjm(), off, adj, lab, threads=threads, limit=4_000_000_000)
+            dt = time.perf_counter() - t
+            total_emb += cnt
+            total_s += dt
+            parts.append(f"{name}: {'first ' if stopped else 'all '}{cnt} embeddings")
+            if not stopped:
+                checks[name] = cnt
+        else:
+            t = time.perf_counter()
+            probe, _, _ = O.match_partial(q.labels, q.adjm(), off, adj, lab, threads=threads, limit=20_000_000)
+            rate = probe / max(time.perf_counter() - t, 1e-6)
+            budget = max(seconds_target - total_s, 5.0)
+            limit = int(min(rate * budget, 50_000_000_000))
+            t = time.perf_counter()
+            cnt, _, stopped = O.match_partial(q.labels, q.adjm(), off, adj, lab, threads=threads, limit=limit)
+            dt = time.perf_counter() - t
+            total_emb += cnt
+            total_s += dt
+            parts.append(f"{name}: {'first ' if stopped else 'all '}{cnt} embeddings")
+            if not stopped:
+                checks[name] = cnt
+    return {"value": total_emb / total_s, "unit": "embeddings/s", "cores": threads, "kind": "port",
+            "sample": "oracle backtracking (oracle/oracle.c), roots handed out dynamically to "
+                      f"{threads} threads, {total_s:.1f} s: " + "; ".join(parts)}, checks
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2", choices=["c2"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--alg", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU: the hot path is single-device; "
+                                   "--gpus N>1 is not supported (see DESIGN.md, sharding)"}))
+        sys.exit(2)
+
+    import numpy as np
+    import fsig_amd as F
+    from fsig_amd import workloads as W
+
+    t0 = time.perf_counter()
+    g = W.c2_graph()
+    queries = W.c2_queries(g)
+    log(f"graph {g.n} V {g.nnz // 2} E {g.num_labels} labels, generated in {time.perf_counter() - t0:.2f}s")
+    ctx = F.Context(0, 64 << 30)
+    dg = ctx.upload(g, pcsr=True)
+    counts = {}
+    for name, q in queries:
+        counts[name] = F.match(dg, q, args.alg, F.MODE_COUNT).count
+    log("counts", counts)
+
+    def step(lat):
+        n = 0
+        for _, q in queries:
+            t = time.perf_counter()
+            n += F.match(dg, q, args.alg, F.MODE_COUNT).count
+            lat.append(time.perf_counter() - t)
+        return n
+
+    for _ in range(args.warmup):
+        step([])
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    lat = []
+    ctx.synchronize()
+    t = time.perf_counter()
+    emb = 0
+    for _ in range(args.steps):
+        emb += step(lat)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t
+    stats = {F.KERNEL_FAMILIES[f]: ctx.kernel_stats(f) for f in range(len(F.KERNEL_FAMILIES))}
+    ctx.set_profiling(False)
+    ms_per_step = elapsed / args.steps * 1e3
+    value = emb / elapsed
+
+    kernels = {}
+    for fam, s in stats.items():
+        if s["launches"]:
+            gbs = s["alg_bytes"] / (s["ms"] / 1e3) / 1e9 if s["ms"] > 0 else 0.0
+            kernels[fam] = {"launches": s["launches"], "ms": round(s["ms"], 3), "alg_GB": round(s["alg_bytes"] / 1e9, 3),
+                            "GB_per_s": round(gbs, 1)}
+    dom = max((f for f in stats if stats[f]["launches"]), key=lambda f: stats[f]["ms"])
+    d = stats[dom]
+    achieved = d["alg_bytes"] / (d["ms"] / 1e3) / 1e9
+    roofline = {"bound": "hbm", "kernel": f"k_join ({dom} pass)", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_ms": round(d["ms"] / d["launches"], 4),
+                "alg_bytes_per_launch": int(d["alg_bytes"] / d["launches"])}
+
+    # alg 2 (GSI path) on the same queries, outside the timed region
+    gsi = {}
+    if args.alg == 1:
+        t = time.perf_counter()
+        n2 = 0
+        for _, q in queries:
+            c2 = F.match(dg, q, F.ALG_GSI, F.MODE_COUNT).count
+            n2 += c2
+            assert c2 == counts[_], "alg 2 disagrees with alg 1"
+        gsi = {"alg2_embeddings_per_s": round(n2 / (time.perf_counter() - t), 1)}
+
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "embeddings/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": "C2 dblp-shaped: Chung-Lu gamma 2.3, 317,080 V, 1,049,866 E, 16 Zipf(1.2) labels; "
+                               "8-vertex dense + sparse random-walk queries; alg 1, count mode",
+                   "graph_seed": W.C2_GRAPH["graph_seed"], "label_seed": W.C2_GRAPH["label_seed"],
+                   "queries": {n: {"edges": q.num_edges, "seed": W.C2_QUERIES[n]["seed"], "embeddings": counts[n]}
+                               for n, q in queries},
+                   "parallelism": "1 GPU"},
+        "queries_per_s": round(len(lat) / elapsed, 2), "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat_ms, 99)), 3), "hbm_frac": roofline["frac"],
+        "roofline": roofline, "kernels": kernels, **gsi,
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cb, checks = cpu_baseline(g, queries)
+        for n, c in checks.items():
+            assert c == counts[n], f"oracle count {c} != engine count {counts[n]} for {n}"
+        out["cpu_baseline"] = cb
+        out["cpu_checked_counts"] = checks
+    print(json.dumps(out), flush=True)
+    dg.free()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -100,7 +100,7 @@ def host_lib():
 
 
 HIP_SYMBOLS = [
-    "fsig_ctx_create", "fsig_ctx_destroy", "fsig_last_error", "fsig_graph_upload_csr",
+    "fsig_ctx_create", "fsig_ctx_destroy", "fsig_last_error", "fsig_ctx_synchronize", "fsig_graph_upload_csr",
     "fsig_graph_upload_pcsr", "fsig_graph_free", "fsig_match", "fsig_set_profiling",
     "fsig_stats_reset", "fsig_kernel_stats", "fsig_last_candidates", "fsig_last_peak_bytes",
     "fsig_set_tune", "fsig_get_tune",
@@ -121,6 +121,7 @@ def hip_lib():
         L = C.CDLL(HIP_LIB_PATH)
         L.fsig_ctx_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(_vp)]
         L.fsig_ctx_destroy.argtypes = [_vp]
+        L.fsig_ctx_synchronize.argtypes = [_vp]
         L.fsig_last_error.restype = C.c_char_p
         L.fsig_last_error.argtypes = [_vp]
         L.fsig_graph_upload_csr.argtypes = [_vp, C.c_uint32, _u32p, _u32p, _i32p, C.c_uint32, _u32p, _u32p,
@@ -337,6 +338,9 @@ class Context:
     def check(self, rc: int):
         if rc:
             raise FsigError(rc, self._L.fsig_last_error(self._h).decode())
+
+    def synchronize(self):
+        self.check(self._L.fsig_ctx_synchronize(self._h))
 
     def upload(self, g: HostGraph, pcsr: bool = False, reference_unit_ids: bool = False) -> "DeviceGraph":
         return DeviceGraph(self, g, pcsr, reference_unit_ids)

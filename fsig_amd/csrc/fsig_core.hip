@@ -171,6 +171,13 @@ void fsig_ctx_destroy(fsig_ctx* c) {
 
 const char* fsig_last_error(const fsig_ctx* c) { return c ? c->err.c_str() : "no context"; }
 
+int fsig_ctx_synchronize(fsig_ctx* c) {
+  if (!c) return FSIG_EINVAL;
+  FSIG_HIP(c, hipSetDevice(c->device));
+  FSIG_HIP(c, hipDeviceSynchronize());
+  return FSIG_OK;
+}
+
 int fsig_graph_upload_csr(fsig_ctx* c, uint32_t n, const uint32_t* off, const uint32_t* adj,
                           const int32_t* labels, uint32_t num_labels, const uint32_t* inv_index,
                           const uint32_t* inv_list, fsig_graph** out) {

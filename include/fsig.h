@@ -51,6 +51,8 @@ enum { FSIG_MODE_COUNT = 0, FSIG_MODE_ENUMERATE = 1 };
 int fsig_ctx_create(int device, uint64_t hbm_budget_bytes, fsig_ctx** out);
 void fsig_ctx_destroy(fsig_ctx* ctx);
 const char* fsig_last_error(const fsig_ctx* ctx);
+/* Waits for all work on the context's device (timing brackets). */
+int fsig_ctx_synchronize(fsig_ctx* ctx);
 
 /* Uploads a data graph in CSR form; it stays resident until fsig_graph_free.
  * off[n+1], adj[off[n]] strictly ascending per vertex (no duplicates),

@@ -48,6 +48,8 @@ def lib():
         L.oracle_match.restype = C.c_uint64
         L.oracle_match.argtypes = [C.c_uint32, _i32p, _u32p, C.c_uint32, _u32p, _u32p, _i32p, C.c_uint32,
                                    _u32p, C.c_uint64, _u64p]
+        L.oracle_match_limit.restype = C.c_uint64
+        L.oracle_match_limit.argtypes = L.oracle_match.argtypes + [C.c_uint64, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -112,18 +114,32 @@ def refine(qlabels, qadjm, off, adj, labels, passes: int, want_sets: bool = Fals
     return (sizes, sets.reshape(k, n)) if want_sets else sizes
 
 
-def match(qlabels, qadjm, off, adj, labels, threads: int = 1, cap: int = 0) -> Tuple[int, Optional[np.ndarray]]:
+TOO_MANY = (1 << 64) - 1
+
+
+def match(qlabels, qadjm, off, adj, labels, threads: int = 1, cap: int = 0,
+          limit: int = 0) -> Tuple[int, Optional[np.ndarray]]:
     """(count, rows) of the injective label- and edge-preserving maps; rows
-    (query-vertex order) only when cap > 0, at most cap of them."""
+    (query-vertex order) only when cap > 0, at most cap of them.  With
+    limit > 0 the search stops after about `limit` embeddings and the count
+    is TOO_MANY (match_partial returns the partial count)."""
+    cnt, rows, stopped = match_partial(qlabels, qadjm, off, adj, labels, threads, cap, limit)
+    return (TOO_MANY if stopped else cnt), rows
+
+
+def match_partial(qlabels, qadjm, off, adj, labels, threads: int = 1, cap: int = 0, limit: int = 0):
+    """(count, rows, stopped): like match(), but a stopped search returns the
+    embeddings it had counted when it stopped."""
     off, adj, labels = _csr(off, adj, labels)
     ql = np.ascontiguousarray(qlabels, dtype=np.int32)
     qa = np.ascontiguousarray(qadjm, dtype=np.uint32)
     k = len(ql)
     rows = np.empty((cap, k), np.uint32) if cap else None
     written = C.c_uint64(0)
-    cnt = lib().oracle_match(k, _p(ql, _i32p), _p(qa), len(labels), _p(off), _p(adj), _p(labels, _i32p), threads,
-                             _p(rows), cap, C.byref(written))
-    return int(cnt), (rows[: written.value] if cap else None)
+    stopped = C.c_int(0)
+    cnt = lib().oracle_match_limit(k, _p(ql, _i32p), _p(qa), len(labels), _p(off), _p(adj), _p(labels, _i32p),
+                                   threads, _p(rows), cap, C.byref(written), limit, C.byref(stopped))
+    return int(cnt), (rows[: written.value] if cap else None), bool(stopped.value)
 
 
 # ---------------------------------------------------------------------------

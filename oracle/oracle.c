@@ -260,6 +260,10 @@ typedef struct {
   uint32_t next_root;
   uint32_t nroots;
   uint32_t* roots;
+  /* abort limit: stop once this many embeddings were seen (0 = none) */
+  uint64_t limit;
+  uint64_t seen;
+  int stop;
 } Match;
 
 static int has_edge(const Match* M, uint32_t a, uint32_t b) {
@@ -292,6 +296,10 @@ static void flush(Worker* W) {
 static void emit(Worker* W) {
   Match* M = W->M;
   W->count++;
+  if (M->limit && (W->count & 1023u) == 0) {
+    uint64_t seen = __atomic_add_fetch(&M->seen, 1024, __ATOMIC_RELAXED);
+    if (seen > M->limit) __atomic_store_n(&M->stop, 1, __ATOMIC_RELAXED);
+  }
   if (!M->rows) return;
   uint32_t* r = W->buf + W->nbuf * M->k;
   for (uint32_t p = 0; p < M->k; p++) r[M->order[p]] = W->phi[p];
@@ -301,6 +309,7 @@ static void emit(Worker* W) {
 static void extend(Worker* W, uint32_t p) {
   Match* M = W->M;
   if (p == M->k) { emit(W); return; }
+  if (M->limit && __atomic_load_n(&M->stop, __ATOMIC_RELAXED)) return;
   const uint32_t u = M->order[p];
   /* iterate the neighbours of the backward neighbour with the smallest image degree */
   uint32_t pivot = OINVALID, best = OINVALID;
@@ -332,7 +341,7 @@ static void* worker_main(void* arg) {
   Match* M = W->M;
   for (;;) {
     uint32_t r = __atomic_fetch_add(&M->next_root, 1, __ATOMIC_RELAXED);
-    if (r >= M->nroots) break;
+    if (r >= M->nroots || (M->limit && __atomic_load_n(&M->stop, __ATOMIC_RELAXED))) break;
     W->phi[0] = M->roots[r];
     extend(W, 1);
   }
@@ -342,10 +351,12 @@ static void* worker_main(void* arg) {
 
 /* Counts (and, when rows != NULL, lists up to cap rows of) the embeddings of
  * the query (k vertices, adjacency bitmasks qadjm) in the data graph.  With
- * threads > 1 the roots are handed out one at a time to a pthread pool. */
-uint64_t oracle_match(uint32_t k, const int32_t* qlabels, const uint32_t* qadjm, uint32_t n, const uint32_t* off,
-                      const uint32_t* adj, const int32_t* labels, uint32_t threads, uint32_t* rows, uint64_t cap,
-                      uint64_t* written) {
+ * threads > 1 the roots are handed out one at a time to a pthread pool.
+ * With limit > 0 the search stops once more than `limit` embeddings were
+ * seen; *stopped tells whether it did (the count is then partial). */
+uint64_t oracle_match_limit(uint32_t k, const int32_t* qlabels, const uint32_t* qadjm, uint32_t n,
+                            const uint32_t* off, const uint32_t* adj, const int32_t* labels, uint32_t threads,
+                            uint32_t* rows, uint64_t cap, uint64_t* written, uint64_t limit, int* stopped) {
   if (k == 0 || k > OMAXQ) return 0;
   Match M;
   memset(&M, 0, sizeof M);
@@ -358,6 +369,7 @@ uint64_t oracle_match(uint32_t k, const int32_t* qlabels, const uint32_t* qadjm,
   M.labels = labels;
   M.rows = rows;
   M.cap = rows ? cap : 0;
+  M.limit = limit;
   for (uint32_t u = 0; u < k; u++) M.qdeg[u] = (uint32_t)__builtin_popcount(qadjm[u]);
   /* order: max degree first, then the vertex with most ordered neighbours */
   uint32_t used = 0;
@@ -393,10 +405,17 @@ uint64_t oracle_match(uint32_t k, const int32_t* qlabels, const uint32_t* qadjm,
   }
   uint64_t total = 0;
   for (uint32_t t = 0; t < threads; t++) total += W[t].count;
+  if (stopped) *stopped = M.stop;
   if (written) *written = M.written;
   pthread_mutex_destroy(&M.lock);
   free(W);
   free(th);
   free(M.roots);
   return total;
+}
+
+uint64_t oracle_match(uint32_t k, const int32_t* qlabels, const uint32_t* qadjm, uint32_t n, const uint32_t* off,
+                      const uint32_t* adj, const int32_t* labels, uint32_t threads, uint32_t* rows, uint64_t cap,
+                      uint64_t* written) {
+  return oracle_match_limit(k, qlabels, qadjm, n, off, adj, labels, threads, rows, cap, written, 0, NULL);
 }

@@ -66,8 +66,8 @@ def test_random_walk_queries_labeled(gpu_ctx, seed):
                                  max_attempts=20000)
             except F.FsigError:
                 continue
-            cnt, _ = O.match(q.labels, q.adjm(), off, adj, lab, threads=8)
-            if cnt > 2_000_000:
+            cnt, _ = O.match(q.labels, q.adjm(), off, adj, lab, threads=8, limit=2_000_000)
+            if cnt == O.TOO_MANY:
                 continue
             for alg in (F.ALG_MMATCH, F.ALG_GSI):
                 _check_set(dg, q, alg, off, adj, lab)
@@ -133,9 +133,18 @@ def test_edge_cases(gpu_ctx):
     k = 12
     qk = F.Query(k, [0] * k, [(a, b) for a in range(k) for b in range(a + 1, k)])
     assert F.match(dg, qk, F.ALG_MMATCH).count == O.match(qk.labels, qk.adjm(), off, adj, lab, threads=8)[0]
-    # invalid query: self-loop, out-of-range vertex
-    with pytest.raises(F.FsigError):
-        F.match(dg, F.Query(2, [0, 0], [(0, 0)]), F.ALG_MMATCH)
+    # malformed queries straight through the C ABI: self-loop, neighbour out
+    # of range, asymmetric adjacency, too many vertices
+    L = gpu_ctx._L
+    cnt = F.C.c_uint64(7)
+    labels = np.zeros(33, np.int32)
+    for qoff, qadj, k in (([0, 1, 2], [0, 0], 2), ([0, 1, 2], [5, 0], 2), ([0, 1, 1], [1], 2),
+                          (list(range(34)), list(range(1, 33)) + [0], 33)):
+        qo = np.asarray(qoff, np.uint32)
+        qa = np.asarray(qadj, np.uint32)
+        rc = L.fsig_match(gpu_ctx._h, dg._h, k, F._ptr(labels, F._i32p), F._ptr(qo), F._ptr(qa),
+                          F._ptr(None), F.ALG_MMATCH, F.MODE_COUNT, 0, F.C.byref(cnt), F._ptr(None), 0)
+        assert rc in (F.FSIG_EINVAL, F.FSIG_EUNSUPPORTED), (qoff, qadj, rc)
 
 
 def test_bounded_enumeration(gpu_ctx):
