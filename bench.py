@@ -79,15 +79,23 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c2"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--alg", type=int, default=1)
+    ap.add_argument("--copy-only", type=int, default=0,
+                    help="only run the STREAM copy kernel over this many bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU: the hot path is single-device; "
-                                   "--gpus N>1 is not supported (see DESIGN.md, sharding)"}))
+                                   "--gpus N>1 is not supported"}))
         sys.exit(2)
 
     import numpy as np
     import fsig_amd as F
     from fsig_amd import workloads as W
+
+    if args.copy_only:
+        ctx = F.Context(0, 2 * args.copy_only + (64 << 20))
+        gbps = ctx.stream_copy(args.copy_only, 5)
+        print(json.dumps({"stream_copy_bytes": args.copy_only, "iters": 6, "GB_per_s": round(gbps, 1)}))
+        return
 
     t0 = time.perf_counter()
     g = W.c2_graph()
@@ -150,6 +158,7 @@ def main():
             assert c2 == counts[_], "alg 2 disagrees with alg 1"
         gsi = {"alg2_embeddings_per_s": round(n2 / (time.perf_counter() - t), 1)}
 
+    stream_gbps = ctx.stream_copy(1 << 30, 20)
     lat_ms = np.array(lat) * 1e3
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "embeddings/s", "n_gpus": 1, "steps": args.steps,
@@ -163,7 +172,7 @@ def main():
                    "parallelism": "1 GPU"},
         "queries_per_s": round(len(lat) / elapsed, 2), "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3), "hbm_frac": roofline["frac"],
-        "roofline": roofline, "kernels": kernels, **gsi,
+        "roofline": roofline, "stream_copy_GB_per_s": round(stream_gbps, 1), "kernels": kernels, **gsi,
     }
     if not args.no_cpu:
         log("cpu baseline ...")

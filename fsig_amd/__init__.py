@@ -103,7 +103,7 @@ HIP_SYMBOLS = [
     "fsig_ctx_create", "fsig_ctx_destroy", "fsig_last_error", "fsig_ctx_synchronize", "fsig_graph_upload_csr",
     "fsig_graph_upload_pcsr", "fsig_graph_free", "fsig_match", "fsig_set_profiling",
     "fsig_stats_reset", "fsig_kernel_stats", "fsig_last_candidates", "fsig_last_peak_bytes",
-    "fsig_set_tune", "fsig_get_tune",
+    "fsig_set_tune", "fsig_get_tune", "fsig_stream_copy",
 ]
 
 
@@ -139,6 +139,7 @@ def hip_lib():
         L.fsig_last_peak_bytes.argtypes = [_vp]
         L.fsig_set_tune.argtypes = [_vp, C.POINTER(Tune)]
         L.fsig_get_tune.argtypes = [_vp, C.POINTER(Tune)]
+        L.fsig_stream_copy.argtypes = [_vp, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]
         _optional_bindings(L)
         _hip = L
     return _hip
@@ -370,6 +371,11 @@ class Context:
         out = np.zeros(k, np.uint32)
         self.check(self._L.fsig_last_candidates(self._h, k, _ptr(out)))
         return out
+
+    def stream_copy(self, nbytes: int, iters: int = 20) -> float:
+        g = C.c_double()
+        self.check(self._L.fsig_stream_copy(self._h, nbytes, iters, C.byref(g)))
+        return g.value
 
     def last_peak_bytes(self) -> int:
         return int(self._L.fsig_last_peak_bytes(self._h))

@@ -66,6 +66,13 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_scan_tiles(const uint32_t* __res
   }
 }
 
+// Plain copy, 16 bytes per thread per step (the STREAM "copy" kernel).
+__global__ void __launch_bounds__(FSIG_BLOCK) k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                            uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 }  // namespace
 
 int scan_u32_to_u64(fsig_ctx* c, const uint32_t* in, uint64_t* out, uint64_t n) {
@@ -332,6 +339,31 @@ int fsig_match(fsig_ctx* c, const fsig_graph* g, uint32_t k, const int32_t* qlab
   if (e != hipSuccess) return set_err(c, FSIG_EHIP, std::string("match: ") + hipGetErrorString(e));
   int rt = resolve_timings(c);
   return rc ? rc : rt;
+}
+
+int fsig_stream_copy(fsig_ctx* c, uint64_t bytes, uint32_t iters, double* gbps) {
+  if (!c || !gbps || bytes < 16 || iters == 0) return set_err(c, FSIG_EINVAL, "stream_copy: bad argument");
+  FSIG_HIP(c, hipSetDevice(c->device));
+  const uint64_t n = bytes / 16;
+  const size_t mk = c->arena.mark();
+  uint4* a = (uint4*)c->arena.alloc(n * 16);
+  uint4* b = (uint4*)c->arena.alloc(n * 16);
+  if (!a || !b) {
+    c->arena.release(mk);
+    return set_err(c, FSIG_ENOMEM, "stream_copy: arena too small");
+  }
+  FSIG_HIP(c, hipMemsetAsync(a, 1, n * 16, c->stream));
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((n + FSIG_BLOCK - 1) / FSIG_BLOCK, (uint64_t)c->num_cus * 32);
+  k_stream_copy<<<grid, FSIG_BLOCK, 0, c->stream>>>(a, b, n);  // warm
+  FSIG_HIP(c, hipEventRecord(c->ev_a, c->stream));
+  for (uint32_t i = 0; i < iters; i++) k_stream_copy<<<grid, FSIG_BLOCK, 0, c->stream>>>(a, b, n);
+  FSIG_HIP(c, hipEventRecord(c->ev_b, c->stream));
+  FSIG_HIP(c, hipEventSynchronize(c->ev_b));
+  float ms = 0;
+  FSIG_HIP(c, hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
+  c->arena.release(mk);
+  *gbps = 2.0 * (double)(n * 16) * iters / (ms * 1e-3) / 1e9;
+  return FSIG_OK;
 }
 
 int fsig_set_profiling(fsig_ctx* c, int on) {

@@ -138,6 +138,13 @@ typedef struct fsig_tune {
 int fsig_set_tune(fsig_ctx* ctx, const fsig_tune* t);
 int fsig_get_tune(const fsig_ctx* ctx, fsig_tune* t);
 
+/* Achievable-bandwidth probe: a plain copy kernel (16-byte loads and stores)
+ * of `bytes` from one arena buffer to another, run `iters` times on the
+ * context stream; *gbps = 2*bytes*iters / elapsed.  Also the known-size
+ * kernel the HBM byte counters are scaled by.  No counterpart in the
+ * reference. */
+int fsig_stream_copy(fsig_ctx* ctx, uint64_t bytes, uint32_t iters, double* gbps);
+
 #ifdef __cplusplus
 }
 #endif
