@@ -103,7 +103,7 @@ HIP_SYMBOLS = [
     "fsig_ctx_create", "fsig_ctx_destroy", "fsig_last_error", "fsig_ctx_synchronize", "fsig_graph_upload_csr",
     "fsig_graph_upload_pcsr", "fsig_graph_free", "fsig_match", "fsig_set_profiling",
     "fsig_stats_reset", "fsig_kernel_stats", "fsig_last_candidates", "fsig_last_peak_bytes",
-    "fsig_set_tune", "fsig_get_tune", "fsig_stream_copy",
+    "fsig_set_tune", "fsig_get_tune", "fsig_stream_copy", "fsig_match_batch", "fsig_last_batch_tune",
 ]
 
 
@@ -147,9 +147,9 @@ def hip_lib():
 
 def _optional_bindings(L):
     """Batched and served entry points (see include/fsig.h)."""
-    if hasattr(L, "fsig_match_batch"):
-        L.fsig_match_batch.argtypes = [_vp, _vp, C.c_uint32, _u32p, C.POINTER(_i32p), C.POINTER(_u32p),
-                                       C.POINTER(_u32p), _u64p, C.POINTER(C.c_double)]
+    L.fsig_match_batch.argtypes = [_vp, _vp, C.c_uint32, _u32p, C.POINTER(_i32p), C.POINTER(_u32p),
+                                   C.POINTER(_u32p), _u64p, C.POINTER(C.c_double)]
+    L.fsig_last_batch_tune.argtypes = [_vp, _u32p, _u32p]
 
 
 # ---------------------------------------------------------------------------
@@ -448,6 +448,29 @@ def match(dg: DeviceGraph, q: Query, alg: int = ALG_MMATCH, mode: int = MODE_COU
         res.embeddings = out[: min(cnt.value, emb_cap)]
     res.candidates = ctx.last_candidates(q.k)
     return res
+
+
+def match_batch(dg: "DeviceGraph", queries: Sequence[Query]) -> Tuple[np.ndarray, float]:
+    """fsig_match_batch: counts of every query (alg 1 semantics) and the
+    persistent join's device time in ms."""
+    ctx = dg.ctx
+    nq = len(queries)
+    ks = np.array([q.k for q in queries], np.uint32)
+    csrs = [q.csr() for q in queries]
+    labs = [np.ascontiguousarray(q.labels, dtype=np.int32) for q in queries]
+    lp = (_i32p * nq)(*[_ptr(l, _i32p) for l in labs])
+    op = (_u32p * nq)(*[_ptr(o) for o, _ in csrs])
+    ap = (_u32p * nq)(*[_ptr(a if len(a) else np.zeros(1, np.uint32)) for _, a in csrs])
+    counts = np.zeros(nq, np.uint64)
+    jm = C.c_double(0)
+    ctx.check(ctx._L.fsig_match_batch(ctx._h, dg._h, nq, _ptr(ks), lp, op, ap, _ptr(counts, _u64p), C.byref(jm)))
+    return counts, jm.value
+
+
+def last_batch_tune(ctx: "Context") -> Tuple[int, int]:
+    g, t = C.c_uint32(), C.c_uint32()
+    ctx.check(ctx._L.fsig_last_batch_tune(ctx._h, C.byref(g), C.byref(t)))
+    return g.value, t.value
 
 
 class GPUSubgraphMatcher:

@@ -113,7 +113,7 @@ int resolve_timings(fsig_ctx* c) {
 }
 
 int read_u64(fsig_ctx* c, const void* dev, uint64_t* host, size_t count) {
-  if (count > 4096) return set_err(c, FSIG_EINVAL, "read_u64: too many words");
+  if (count > kPinnedWords) return set_err(c, FSIG_EINVAL, "read_u64: too many words");
   FSIG_HIP(c, hipMemcpyAsync(c->h_pinned, dev, count * 8, hipMemcpyDeviceToHost, c->stream));
   FSIG_HIP(c, hipStreamSynchronize(c->stream));
   std::memcpy(host, c->h_pinned, count * 8);
@@ -142,7 +142,7 @@ int fsig_ctx_create(int device, uint64_t budget, fsig_ctx** out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess ||
       hipMalloc((void**)&c->d_counters, 64 * sizeof(unsigned long long)) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_pinned, 4096 * 8, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c->h_pinned, kPinnedWords * 8, hipHostMallocDefault) != hipSuccess) {
     fsig_ctx_destroy(c);
     return FSIG_EHIP;
   }
@@ -395,6 +395,13 @@ int fsig_last_candidates(const fsig_ctx* c, uint32_t k, uint32_t* sizes) {
 }
 
 uint64_t fsig_last_peak_bytes(const fsig_ctx* c) { return c ? c->last_peak : 0; }
+
+int fsig_last_batch_tune(const fsig_ctx* c, uint32_t* lanes, uint32_t* tile) {
+  if (!c) return FSIG_EINVAL;
+  if (lanes) *lanes = c->last_tune_G;
+  if (tile) *tile = c->last_tune_lds;
+  return FSIG_OK;
+}
 
 int fsig_set_tune(fsig_ctx* c, const fsig_tune* t) {
   if (!c || !t) return FSIG_EINVAL;

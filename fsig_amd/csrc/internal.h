@@ -10,6 +10,7 @@
 #include "../../include/fsig.h"
 
 #define MAXQ 32
+constexpr size_t kPinnedWords = 65536;
 #define FSIG_INVALID 0xffffffffu
 
 // ---------------------------------------------------------------------------
@@ -70,8 +71,11 @@ struct fsig_ctx {
   // kernel family f accumulated device-side statistics ([0] algorithmic
   // bytes, [1] units).  Host staging is pinned.
   unsigned long long* d_counters = nullptr;
-  unsigned long long* h_pinned = nullptr;  // 4096 words of pinned staging
+  unsigned long long* h_pinned = nullptr;  // kPinnedWords words of pinned staging
   int num_cus = 256;
+  // choice of the batched engine's autotuner on its last batch
+  uint32_t last_tune_G = 0;
+  uint32_t last_tune_lds = 0;
 };
 
 struct UnitMeta {

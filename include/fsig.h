@@ -98,6 +98,21 @@ int fsig_match(fsig_ctx* ctx, const fsig_graph* g, uint32_t k, const int32_t* ql
                const uint32_t* qoff, const uint32_t* qadj, const uint32_t* qsig, int alg, int mode,
                uint64_t max_embeddings, uint64_t* count_out, uint32_t* emb_out, uint64_t emb_cap);
 
+/* Counts the embeddings of nq queries (alg 1 semantics) against one
+ * resident graph.  Filter, refinement and candidate edges run for the whole
+ * batch at once; the join is one persistent kernel, launched as a captured
+ * hipGraph, whose groups take (query, root range) tasks from a device work
+ * queue and walk them depth-first.  An autotuner times the candidate
+ * configurations (lanes per group, LDS staging, tile) on a sample of the
+ * queue unless fsig_set_tune fixed lanes_per_row.  counts_out[i] is query
+ * i's count; *join_ms (may be NULL) the persistent kernel's device time.
+ * No counterpart in the reference (it runs one query per process). */
+int fsig_match_batch(fsig_ctx* ctx, const fsig_graph* g, uint32_t nq, const uint32_t* k,
+                     const int32_t* const* qlabels, const uint32_t* const* qoff,
+                     const uint32_t* const* qadj, uint64_t* counts_out, double* join_ms);
+/* Configuration the autotuner chose for the last batch (lds_tile 0: none). */
+int fsig_last_batch_tune(const fsig_ctx* ctx, uint32_t* lanes_per_row, uint32_t* lds_tile);
+
 /* Per-kernel-family statistics of the context, accumulated since the last
  * fsig_stats_reset.  With profiling on, every launch of the family is
  * bracketed by HIP events on the context stream and its time is summed. */

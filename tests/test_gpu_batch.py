@@ -1,0 +1,67 @@
+"""The batched engine (fsig_match_batch): every query's count equals its
+plain fsig_match count and the oracle's, for a mix of sizes, densities,
+disconnected queries, absent labels and empty candidate sets."""
+import numpy as np
+import pytest
+
+import fsig_amd as F
+import oracle as O
+from fsig_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_queries(g, n, seed):
+    qs = [q for _, q in W.walk_queries(g, n, 3, 9, seed)]
+    qs.append(F.Query(1, [2], np.zeros((0, 2))))                      # single vertex
+    qs.append(F.Query(4, [0, 1, 0, 1], [(0, 1), (2, 3)]))             # disconnected
+    qs.append(F.Query(3, [0, 99, 1], [(0, 1), (1, 2)]))               # label absent from the graph
+    k = 9
+    qs.append(F.Query(k, [0] * k, [(a, b) for a in range(k) for b in range(a + 1, k)]))  # no host
+    qs.append(F.Query(2, [1, 1], [(0, 1)]))
+    return qs
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_batch_equals_plain_and_oracle(gpu_ctx, seed):
+    g = F.HostGraph.chung_lu(20000, 100000, 2.3, 90 + seed).set_labels("zipf", 8, 1.2, 95 + seed)
+    off, adj, lab = g.arrays()
+    dg = gpu_ctx.upload(g)
+    qs = _mixed_queries(g, 40, seed)
+    counts, join_ms = F.match_batch(dg, qs)
+    assert join_ms >= 0
+    for q, c in zip(qs, counts.tolist()):
+        plain = F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT).count
+        assert c == plain, (q.k, q.labels.tolist(), q.edges.tolist())
+        want, _ = O.match(q.labels, q.adjm(), off, adj, lab, threads=8, limit=20_000_000)
+        if want != O.TOO_MANY:
+            assert c == want
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("strategy", [1, 2])
+def test_batch_configurations_agree(gpu_ctx, lanes, strategy):
+    g = F.HostGraph.chung_lu(15000, 90000, 2.3, 7).set_labels("uniform", 4, seed=8)
+    dg = gpu_ctx.upload(g)
+    qs = [q for _, q in W.walk_queries(g, 24, 3, 8, 11)]
+    ref = [F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT).count for q in qs]
+    try:
+        gpu_ctx.set_tune(lanes_per_row=lanes, strategy=strategy, lds_tile=128)
+        counts, _ = F.match_batch(dg, qs)
+        assert counts.tolist() == ref
+        assert F.last_batch_tune(gpu_ctx)[0] == lanes
+    finally:
+        gpu_ctx.set_tune()
+
+
+def test_batch_autotuner_and_repeat(gpu_ctx):
+    g = F.HostGraph.chung_lu(30000, 200000, 2.3, 12).set_labels("zipf", 16, 1.2, 13)
+    dg = gpu_ctx.upload(g)
+    qs = [q for _, q in W.walk_queries(g, 64, 4, 12, 14)]
+    a, _ = F.match_batch(dg, qs)
+    b, _ = F.match_batch(dg, qs)   # second call reuses the tuning decision
+    assert a.tolist() == b.tolist()
+    lanes, tile = F.last_batch_tune(gpu_ctx)
+    assert lanes in (4, 8, 16, 32, 64)
+    ref = [F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT).count for q in qs]
+    assert a.tolist() == ref
