@@ -230,7 +230,7 @@ def bench_c4(args):
     ctx.synchronize()
     batch_s = (time.perf_counter() - t) / args.steps
     st = ctx.kernel_stats(F.K_BATCH)
-    lanes, tile = F.last_batch_tune(ctx)
+    lanes, tile, engine = F.last_batch_tune(ctx)
     lat_ms = np.array(lat) * 1e3
     emb = int(sum(plain_counts))
     out = {"metric": METRIC, "value": round(len(queries) / batch_s, 2), "unit": "queries/s", "n_gpus": 1,
@@ -246,7 +246,7 @@ def bench_c4(args):
            "plain_p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
            "persistent_join_ms": round(float(np.mean(join_ms)), 3),
            "persistent_join_alg_GB_per_s": round(st["alg_bytes"] / (st["ms"] / 1e3) / 1e9, 1) if st["ms"] else None,
-           "autotuned": {"lanes_per_group": lanes, "lds_tile_words": tile}}
+           "autotuned": {"engine": engine, "lanes": lanes, "lds_tile_words": tile}}
     print(json.dumps(out), flush=True)
 
 

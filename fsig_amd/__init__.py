@@ -27,6 +27,7 @@ HOST_LIB_PATH = os.path.join(PKG_DIR, "libfsig_host.so")
 FSIG_BIN = os.path.join(PKG_DIR, "bin", "FSIG")
 
 FSIG_OK, FSIG_EINVAL, FSIG_ENOMEM, FSIG_EHIP, FSIG_EUNSUPPORTED, FSIG_ENOTFOUND, FSIG_EAGAIN = 0, -1, -2, -3, -4, -5, -6
+FSIG_ELIMIT = -7
 ALG_MMATCH, ALG_GSI = 1, 2
 MODE_COUNT, MODE_ENUMERATE = 0, 1
 K_FILTER, K_REFINE, K_CEDGE, K_JOIN_COUNT, K_JOIN_WRITE, K_SCAN, K_BATCH = range(7)
@@ -104,6 +105,7 @@ HIP_SYMBOLS = [
     "fsig_graph_upload_pcsr", "fsig_graph_free", "fsig_match", "fsig_set_profiling",
     "fsig_stats_reset", "fsig_kernel_stats", "fsig_last_candidates", "fsig_last_peak_bytes",
     "fsig_set_tune", "fsig_get_tune", "fsig_stream_copy", "fsig_match_batch", "fsig_last_batch_tune",
+    "fsig_set_work_limit",
 ]
 
 
@@ -140,6 +142,7 @@ def hip_lib():
         L.fsig_set_tune.argtypes = [_vp, C.POINTER(Tune)]
         L.fsig_get_tune.argtypes = [_vp, C.POINTER(Tune)]
         L.fsig_stream_copy.argtypes = [_vp, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]
+        L.fsig_set_work_limit.argtypes = [_vp, C.c_uint64]
         _optional_bindings(L)
         _hip = L
     return _hip
@@ -149,7 +152,7 @@ def _optional_bindings(L):
     """Batched and served entry points (see include/fsig.h)."""
     L.fsig_match_batch.argtypes = [_vp, _vp, C.c_uint32, _u32p, C.POINTER(_i32p), C.POINTER(_u32p),
                                    C.POINTER(_u32p), _u64p, C.POINTER(C.c_double)]
-    L.fsig_last_batch_tune.argtypes = [_vp, _u32p, _u32p]
+    L.fsig_last_batch_tune.argtypes = [_vp, _u32p, _u32p, _u32p]
 
 
 # ---------------------------------------------------------------------------
@@ -372,6 +375,9 @@ class Context:
         self.check(self._L.fsig_last_candidates(self._h, k, _ptr(out)))
         return out
 
+    def set_work_limit(self, max_rows: int):
+        self.check(self._L.fsig_set_work_limit(self._h, max_rows))
+
     def stream_copy(self, nbytes: int, iters: int = 20) -> float:
         g = C.c_double()
         self.check(self._L.fsig_stream_copy(self._h, nbytes, iters, C.byref(g)))
@@ -467,10 +473,14 @@ def match_batch(dg: "DeviceGraph", queries: Sequence[Query]) -> Tuple[np.ndarray
     return counts, jm.value
 
 
-def last_batch_tune(ctx: "Context") -> Tuple[int, int]:
-    g, t = C.c_uint32(), C.c_uint32()
-    ctx.check(ctx._L.fsig_last_batch_tune(ctx._h, C.byref(g), C.byref(t)))
-    return g.value, t.value
+BATCH_ENGINES = {1: "lockstep", 2: "persistent"}
+
+
+def last_batch_tune(ctx: "Context") -> Tuple[int, int, str]:
+    """(lanes per row/group, LDS tile words, engine) of the last batch."""
+    g, t, e = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    ctx.check(ctx._L.fsig_last_batch_tune(ctx._h, C.byref(g), C.byref(t), C.byref(e)))
+    return g.value, t.value, BATCH_ENGINES.get(e.value, "none")
 
 
 class GPUSubgraphMatcher:

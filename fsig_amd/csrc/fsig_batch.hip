@@ -17,6 +17,9 @@
 // for each candidate (lanes per group, intersection strategy, LDS tile) and
 // runs the batch with the fastest; choices are cached per batch shape.
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 
@@ -158,6 +161,7 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_brefine(const BClass* __restrict
                                                         const uint32_t* __restrict__ adj,
                                                         uint8_t* __restrict__ keep) {
   constexpr int GPB = FSIG_BLOCK / G;
+  __shared__ uint8_t rem[GPB][MAXQ];
   const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
   const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
   const uint64_t ngroups = (uint64_t)gridDim.x * GPB;
@@ -166,21 +170,22 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_brefine(const BClass* __restrict
     const uint32_t v = cl.cand[f - cl.flat];
     const uint32_t s = off[v], e = off[v + 1];
     const uint32_t* mask = cl.mask;
-    bool ok = true;
-    uint32_t tm = cl.tmask;
-    while (tm && ok) {
-      const uint32_t b = __builtin_ctz(tm);
-      tm &= tm - 1;
-      const uint32_t need = thr[(uint64_t)cl.thr_row * MAXQ + b];
-      uint32_t got = 0;
-      for (uint32_t j = s; j < e && got < need; j += G) {
-        const uint32_t i = j + gl;
-        const bool hit = i < e && ((mask[adj[i]] >> b) & 1u);
-        got += __popcll(__ballot(hit) & gmask);
+    const uint8_t* need = thr + (uint64_t)cl.thr_row * MAXQ;
+    uint32_t open = cl.tmask;
+    for (uint32_t tm = open; tm; tm &= tm - 1) rem[gw][__builtin_ctz(tm)] = need[__builtin_ctz(tm)];
+    for (uint32_t j = s; j < e && open; j += G) {
+      const uint32_t i = j + gl;
+      const uint32_t m = i < e ? (mask[adj[i]] & open) : 0u;
+      for (uint32_t tm = open; tm; tm &= tm - 1) {
+        const uint32_t b = __builtin_ctz(tm);
+        const uint32_t got = __popcll(__ballot((m >> b) & 1u) & gmask);
+        const uint32_t r = rem[gw][b];
+        const uint32_t nr = r > got ? r - got : 0;
+        rem[gw][b] = (uint8_t)nr;
+        if (nr == 0) open &= ~(1u << b);
       }
-      ok = got >= need;
     }
-    if (gl == 0) keep[f] = ok ? 1 : 0;
+    if (gl == 0) keep[f] = open == 0 ? 1 : 0;
   }
 }
 
@@ -200,9 +205,14 @@ struct BPair {
   const uint32_t* bcand;
   const uint32_t* mask;
   uint32_t* off_out;
+  const uint32_t* bpos;  // vertex -> candidate index of b (null: binary search)
   uint32_t asize, bsize, bbit, pad;
   uint64_t flat;
 };
+
+__global__ void k_bscatter_pos(const uint32_t* __restrict__ cand, uint32_t n, uint32_t* __restrict__ pos) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) pos[cand[j]] = j;
+}
 
 template <int G>
 __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_count(const BPair* __restrict__ P, uint32_t np, uint64_t total,
@@ -263,7 +273,7 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_fill(const BPair* __restr
         hit = (pd.mask[x] >> pd.bbit) & 1u;
       }
       const uint64_t bal = __ballot(hit) & gmask;
-      if (hit) lst_all[at + __popcll(bal & below)] = sorted_index(pd.bcand, pd.bsize, x);
+      if (hit) lst_all[at + __popcll(bal & below)] = pd.bpos ? pd.bpos[x] : sorted_index(pd.bcand, pd.bsize, x);
       at += __popcll(bal);
     }
   }
@@ -292,11 +302,19 @@ struct BLevel {
 struct BQuery {
   uint32_t k;
   uint32_t level_base;  // index of this query's level 0 in the level table
+  // Edge tasks: when join position 1 has position 0 as its only backward
+  // neighbour, a task is a range of entries of that candidate-edge list,
+  // i.e. of (root, first neighbour) pairs, so a hub root's subtree is spread
+  // over many tasks.  n1 = roots, off1/lst1 = the list.
+  uint32_t edge_tasks;
+  uint32_t n1;
+  const uint32_t* off1;
+  const uint32_t* lst1;
 };
 
 struct Task {
   uint32_t q;
-  uint32_t r0, r1;  // roots (candidate indices of the first join vertex)
+  uint32_t r0, r1;  // roots, or entries of the first list for edge tasks
 };
 
 // The candidates of level p (the join position p) for the path in `path`:
@@ -341,6 +359,30 @@ __device__ __forceinline__ uint32_t level_candidates(const BLevel& L, const uint
   }
   const uint32_t* dl = L.nb ? L.blst[d] + sh_s[d] : nullptr;
   uint32_t base = 0;
+  if (!STORE && L.nb == 1) {
+    // one backward list: the count is its length minus the earlier vertices
+    // of the same label found in it (binary search, one lane per conflict)
+    uint32_t hit = 0;
+    for (uint32_t j0 = 0; j0 < L.ncf; j0 += G) {
+      const uint32_t j = j0 + gl;
+      bool in = false;
+      if (j < L.ncf) {
+        uint32_t target = sh_cf[j];
+        bool ok = true;
+        if (!((L.cfsame >> j) & 1u)) {
+          const uint32_t at = sorted_index(L.wcand, L.wsize, target);
+          ok = at < L.wsize && L.wcand[at] == target;
+          target = at;
+        }
+        in = ok && sorted_contains(dl, dlen, target);
+      }
+      hit += __popcll(__ballot(in) & gmask);
+    }
+    bytes -= 4ull * el;  // the list itself is not read
+    bytes += 4ull * L.ncf * (32 - __builtin_clz(dlen | 1));
+    if (staged) __builtin_amdgcn_wave_barrier();
+    return dlen - hit;
+  }
   for (uint32_t t0 = 0; t0 < dlen; t0 += G) {
     const uint32_t t = t0 + gl;
     bool ok = t < dlen;
@@ -367,12 +409,123 @@ __device__ __forceinline__ uint32_t level_candidates(const BLevel& L, const uint
   return base;
 }
 
+// Work queue of the persistent join.  Seed tasks (ranges of roots or of
+// first-list entries, built on the host) are taken first; donated tasks are
+// (query, level p, path prefix, candidate range at p) records pushed by busy
+// groups while other groups wait, each taken by one ticket.  `outstanding`
+// counts seed chunks plus donated tasks not finished yet; a waiting group
+// leaves when it reaches 0 (no task exists and none can be created).
+struct DTask {
+  uint32_t q, p, a, b;
+  uint32_t path[MAXQ];
+};
+
+struct Queue {
+  unsigned int seed_head;
+  unsigned int don_head;     // tickets taken
+  unsigned int don_reserve;  // slots reserved by donors
+  unsigned int outstanding;
+  unsigned int error;        // a wait ran past its bound
+  unsigned int pad[3];
+};
+
+constexpr uint64_t kSpinLimit = 1ull << 27;
+constexpr uint32_t kDonateMin = 4;
+
+// The persistent join is a state machine: every iteration of one convergent
+// loop advances each lane group by one bounded step (take a task, one DFS
+// step, or one poll of the queue).  Groups sharing a wavefront therefore
+// interleave; a group waiting for donated work never holds up the wave-mate
+// that would donate it.
+enum : uint32_t { kNeed = 0, kSeedItem = 1, kWalk = 2, kWait = 3, kDone = 4 };
+
+// Count of the last level for one lane's own candidate x of level p (the
+// path is path[0..p-1] followed by x): each lane works alone, so a group
+// counts G leaf parents per step.  Returns false when the level has more
+// backward neighbours than the lane keeps in registers (the caller then uses
+// the group-cooperative count).
+constexpr uint32_t kLaneNb = 6;
+constexpr uint32_t kLaneCf = 8;
+
+__device__ __forceinline__ bool lane_count(const BLevel& L, const uint32_t* path, uint32_t p, uint32_t x,
+                                           uint32_t& out, uint64_t& bytes) {
+  if (L.nb > kLaneNb || L.ncf > kLaneCf) return false;
+  uint32_t s[kLaneNb], n[kLaneNb];
+  const uint32_t* l[kLaneNb];
+  uint32_t d = 0, dn = 0xffffffffu;
+#pragma unroll
+  for (uint32_t i = 0; i < kLaneNb; i++) {
+    if (i < L.nb) {
+      const uint32_t c = L.bcol[i];
+      const uint32_t v = c == p ? x : path[c];
+      const uint32_t* o = L.boff[i];
+      s[i] = o[v];
+      n[i] = o[v + 1] - s[i];
+      l[i] = L.blst[i] + s[i];
+      if (n[i] < dn) { dn = n[i]; d = i; }
+    }
+  }
+  bytes += 8ull * L.nb;
+  if (L.nb == 0) { dn = L.wsize; }
+  // conflicts in the index space of w (data vertex ids mapped by search)
+  uint32_t cnt = 0;
+  if (L.nb == 1) {
+    cnt = dn;
+    for (uint32_t j = 0; j < L.ncf; j++) {
+      const uint32_t c = L.cfcol[j];
+      uint32_t target = c == p ? x : path[c];
+      bool ok = true;
+      if (!((L.cfsame >> j) & 1u)) {
+        const uint32_t id = L.cfcand[j][target];
+        const uint32_t at = sorted_index(L.wcand, L.wsize, id);
+        ok = at < L.wsize && L.wcand[at] == id;
+        target = at;
+      }
+      if (ok && sorted_contains(l[0], dn, target)) cnt--;
+    }
+    bytes += 4ull * L.ncf * (32 - __builtin_clz(dn | 1));
+  } else {
+    uint32_t cf[kLaneCf];
+#pragma unroll
+    for (uint32_t j = 0; j < kLaneCf; j++) {
+      if (j < L.ncf) {
+        const uint32_t c = L.cfcol[j];
+        uint32_t v = c == p ? x : path[c];
+        if (!((L.cfsame >> j) & 1u)) v = L.cfcand[j][v];
+        cf[j] = v;
+      }
+    }
+    for (uint32_t t = 0; t < dn; t++) {
+      const uint32_t y = L.nb ? l[d][t] : t;
+      bool ok = true;
+#pragma unroll
+      for (uint32_t i = 0; i < kLaneNb; i++)
+        if (i < L.nb && i != d && ok) ok = sorted_contains(l[i], n[i], y);
+      if (ok && L.ncf) {
+        const uint32_t yid = L.any_cross ? L.wcand[y] : y;
+#pragma unroll
+        for (uint32_t j = 0; j < kLaneCf; j++)
+          if (j < L.ncf) ok &= (((L.cfsame >> j) & 1u) ? y : yid) != cf[j];
+      }
+      cnt += ok;
+    }
+    uint64_t el = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kLaneNb; i++)
+      if (i < L.nb) el += n[i];
+    bytes += 4ull * el;
+  }
+  out = cnt;
+  return true;
+}
+
 template <int G, bool LDS_STAGE>
 __global__ void __launch_bounds__(FSIG_BLOCK) k_dfs(const BQuery* __restrict__ Q, const BLevel* __restrict__ LV,
                                                     const Task* __restrict__ tasks, uint32_t ntasks,
-                                                    unsigned int* __restrict__ head, uint32_t* __restrict__ stacks,
-                                                    uint32_t stack_words, uint32_t tile_words,
-                                                    unsigned long long* __restrict__ counts,
+                                                    Queue* __restrict__ qu, DTask* __restrict__ dtasks,
+                                                    unsigned int* __restrict__ ready, uint32_t dcap,
+                                                    uint32_t* __restrict__ stacks, uint32_t stack_words,
+                                                    uint32_t tile_words, unsigned long long* __restrict__ counts,
                                                     unsigned long long* __restrict__ stats) {
   constexpr int GPB = FSIG_BLOCK / G;
   __shared__ uint32_t sh_path[GPB][MAXQ];
@@ -391,53 +544,209 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_dfs(const BQuery* __restrict__ Q
   uint32_t* path = sh_path[gw];
   uint32_t* len = sh_len[gw];
   uint32_t* pos = sh_pos[gw];
-  uint64_t bytes = 0, nodes = 0;
-  for (;;) {
-    uint32_t ti = 0;
-    if (gl == 0) ti = atomicAdd(head, 1u);
-    ti = __shfl(ti, 0, G);
-    if (ti >= ntasks) break;  // every group leaves once the queue is drained
-    const Task T = tasks[ti];
-    const BQuery q = Q[T.q];
-    const BLevel* lv = LV + q.level_base;
-    unsigned long long cnt = 0;
-    for (uint32_t root = T.r0; root < T.r1; root++) {
-      path[0] = root;
-      if (q.k == 1) { cnt++; continue; }
-      if (q.k == 2) {
-        cnt += level_candidates<G, false, LDS_STAGE>(lv[1], path, sh_s[gw], sh_n[gw], sh_cf[gw], tile, tile_words,
-                                                     gl, gmask, below, nullptr, bytes);
-        nodes++;
+  uint64_t bytes = 0, nodes = 0, spins = 0;
+  // group state (uniform within the group)
+  uint32_t st = kNeed;
+  bool seeds_left = true;
+  bool from_seed = false;
+  uint32_t qi = 0, k = 0, p0 = 0, p = 0, item = 0, item_end = 0, ticket = 0, since_check = 0;
+  const BLevel* lv = LV;
+  unsigned long long cnt = 0;
+
+  // Starts the walk below path[0..s0-1] over candidates [a, b) of level s0.
+  auto start_walk = [&](uint32_t s0, uint32_t a, uint32_t b) {
+    if (s0 + 1 == k) {  // level s0 is the last one: count it
+      cnt += level_candidates<G, false, LDS_STAGE>(lv[s0], path, sh_s[gw], sh_n[gw], sh_cf[gw], tile, tile_words, gl,
+                                                   gmask, below, nullptr, bytes);
+      return false;
+    }
+    const uint32_t full = level_candidates<G, true, LDS_STAGE>(lv[s0], path, sh_s[gw], sh_n[gw], sh_cf[gw], tile,
+                                                               tile_words, gl, gmask, below, stk + lv[s0].stack_off,
+                                                               bytes);
+    len[s0] = b < full ? b : full;
+    pos[s0] = a;
+    p0 = s0;
+    p = s0;
+    nodes++;
+    return true;
+  };
+  auto finish_task = [&]() {
+    if (gl == 0) {
+      atomicAdd(&counts[qi], cnt);
+      atomicSub(&qu->outstanding, 1u);
+    }
+    cnt = 0;
+    st = kNeed;
+  };
+
+  while (st != kDone) {
+    if (st == kNeed) {
+      if (seeds_left) {
+        uint32_t ti = 0;
+        if (gl == 0) ti = atomicAdd(&qu->seed_head, 1u);
+        ti = __shfl(ti, 0, G);
+        if (ti < ntasks) {
+          const Task T = tasks[ti];
+          qi = T.q;
+          const BQuery q = Q[qi];
+          k = q.k;
+          lv = LV + q.level_base;
+          item = T.r0;
+          item_end = T.r1;
+          cnt = 0;
+          st = kSeedItem;
+        } else {
+          seeds_left = false;
+        }
+      } else {
+        uint32_t t = 0;
+        if (gl == 0) t = atomicAdd(&qu->don_head, 1u);
+        t = __shfl(t, 0, G);
+        if (t >= dcap) {
+          st = kDone;  // no task can ever land in this slot
+        } else {
+          ticket = t;
+          spins = 0;
+          st = kWait;
+        }
+      }
+    } else if (st == kSeedItem) {
+      if (item >= item_end) {
+        finish_task();
+      } else {
+        const BQuery q = Q[qi];
+        uint32_t s0;
+        bool skip = false;
+        if (q.edge_tasks) {
+          uint32_t lo = 0, hi = q.n1;  // root: last r with off1[r] <= item
+          while (hi - lo > 1) {
+            const uint32_t m = (lo + hi) >> 1;
+            if (q.off1[m] <= item) lo = m; else hi = m;
+          }
+          path[0] = lo;
+          path[1] = q.lst1[item];
+          s0 = 2;
+          if (k == 2) { cnt++; skip = true; }
+        } else {
+          path[0] = item;
+          s0 = 1;
+          if (k == 1) { cnt++; skip = true; }
+        }
+        item++;
+        if (!skip && start_walk(s0, 0, 0xffffffffu)) {
+          from_seed = true;
+          since_check = 0;
+          st = kWalk;
+        }
+      }
+    } else if (st == kWait) {
+      // Relaxed polls: an agent-scope acquire invalidates the L2 of the XCD,
+      // so it is issued once, after the flag is seen.
+      unsigned int state = 0;  // 1 ready, 2 leave
+      if (gl == 0) {
+        if (__hip_atomic_load(&ready[ticket], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) state = 1;
+        else if (__hip_atomic_load(&qu->outstanding, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) state = 2;
+        else if (++spins > kSpinLimit) { atomicExch(&qu->error, 1u); state = 2; }
+      }
+      state = __shfl(state, 0, G);
+      if (state == 2) {
+        st = kDone;
+      } else if (state == 1) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const DTask* dt = dtasks + ticket;
+        qi = dt->q;
+        const uint32_t tp = dt->p, a = dt->a, b = dt->b;
+        for (uint32_t c = gl; c < tp; c += G) path[c] = dt->path[c];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const BQuery q = Q[qi];
+        k = q.k;
+        lv = LV + q.level_base;
+        cnt = 0;
+        from_seed = false;
+        since_check = 0;
+        if (start_walk(tp, a, b)) st = kWalk;
+        else finish_task();
+      } else {
+        __builtin_amdgcn_s_sleep(1);
+      }
+    } else {  // kWalk: one step of the depth-first walk
+      if (pos[p] >= len[p]) {
+        if (p == p0) {
+          if (from_seed) st = kSeedItem;
+          else finish_task();
+        } else {
+          p--;
+        }
         continue;
       }
-      len[1] = level_candidates<G, true, LDS_STAGE>(lv[1], path, sh_s[gw], sh_n[gw], sh_cf[gw], tile, tile_words,
-                                                    gl, gmask, below, stk + lv[1].stack_off, bytes);
-      pos[1] = 0;
-      nodes++;
-      uint32_t p = 1;
-      for (;;) {
-        if (pos[p] == len[p]) {
-          if (p == 1) break;
-          p--;
-          continue;
-        }
-        const uint32_t x = stk[lv[p].stack_off + pos[p]];
-        pos[p] = pos[p] + 1;
-        path[p] = x;
-        nodes++;
-        if (p + 2 == q.k) {
-          cnt += level_candidates<G, false, LDS_STAGE>(lv[p + 1], path, sh_s[gw], sh_n[gw], sh_cf[gw], tile,
-                                                       tile_words, gl, gmask, below, nullptr, bytes);
-        } else {
-          len[p + 1] = level_candidates<G, true, LDS_STAGE>(lv[p + 1], path, sh_s[gw], sh_n[gw], sh_cf[gw], tile,
-                                                            tile_words, gl, gmask, below,
-                                                            stk + lv[p + 1].stack_off, bytes);
-          pos[p + 1] = 0;
-          p++;
+      // Donation: while groups wait for work, hand half of the shallowest
+      // level with at least kDonateMin candidates left to the queue (rare and
+      // coarse: each donation costs one agent-scope release).
+      if (++since_check >= 32) {
+        since_check = 0;
+        unsigned int hungry = 0;
+        if (gl == 0)
+          hungry = __hip_atomic_load(&qu->don_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >
+                   __hip_atomic_load(&qu->don_reserve, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        hungry = __shfl(hungry, 0, G);
+        if (hungry) {
+          uint32_t dp = p0;
+          while (dp <= p && len[dp] - pos[dp] < kDonateMin) dp++;
+          if (dp <= p) {
+            unsigned int slot = 0;
+            if (gl == 0) slot = atomicAdd(&qu->don_reserve, 1u);
+            slot = __shfl(slot, 0, G);
+            if (slot < dcap) {
+              const uint32_t mid = pos[dp] + (len[dp] - pos[dp]) / 2;
+              DTask* t = dtasks + slot;
+              for (uint32_t c = gl; c < dp; c += G) t->path[c] = path[c];
+              if (gl == 0) {
+                t->q = qi;
+                t->p = dp;
+                t->a = mid;
+                t->b = len[dp];
+              }
+              len[dp] = mid;
+              if (gl == 0) atomicAdd(&qu->outstanding, 1u);
+              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+              if (gl == 0) __hip_atomic_store(&ready[slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+          }
         }
       }
+      if (p + 2 == k && lv[p + 1].nb <= kLaneNb && lv[p + 1].ncf <= kLaneCf) {
+        // leaf parents: each lane counts the last level for its own candidate
+        const uint32_t take = len[p] - pos[p] < G ? len[p] - pos[p] : G;
+        uint32_t mine = 0;
+        if (gl < take) {
+          const uint32_t x = stk[lv[p].stack_off + pos[p] + gl];
+          lane_count(lv[p + 1], path, p, x, mine, bytes);
+        }
+        unsigned long long sum = mine;
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, G);
+        cnt += sum;
+        pos[p] = pos[p] + take;
+        nodes += take;
+        continue;
+      }
+      const uint32_t x = stk[lv[p].stack_off + pos[p]];
+      pos[p] = pos[p] + 1;
+      path[p] = x;
+      nodes++;
+      if (p + 2 == k) {
+        cnt += level_candidates<G, false, LDS_STAGE>(lv[p + 1], path, sh_s[gw], sh_n[gw], sh_cf[gw], tile,
+                                                     tile_words, gl, gmask, below, nullptr, bytes);
+      } else {
+        len[p + 1] = level_candidates<G, true, LDS_STAGE>(lv[p + 1], path, sh_s[gw], sh_n[gw], sh_cf[gw], tile,
+                                                          tile_words, gl, gmask, below, stk + lv[p + 1].stack_off,
+                                                          bytes);
+        pos[p + 1] = 0;
+        p++;
+      }
     }
-    if (gl == 0) atomicAdd(&counts[T.q], cnt);
   }
   if (stats && gl == 0) {
     atomicAdd(&stats[0], (unsigned long long)bytes);
@@ -445,9 +754,183 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_dfs(const BQuery* __restrict__ Q
   }
 }
 
+// ---------------------------------------------------------------------------
+// Lockstep batched join: level p of every query that still expands is one
+// table (rows of p candidate indices, queries back to back); one count
+// launch, one scan and one write launch serve the whole batch per level.
+// The per-row work is the plain join's (k_join); the level description is
+// looked up per row from the query the row belongs to.
+struct MLevel {
+  JoinLevel L;
+  CandEdgeSrc S;
+  uint32_t last;  // level p+1 is the query's last: count only
+  uint32_t qid;   // batch index of the query
+};
+
+template <int G, bool WRITE>
+__global__ void __launch_bounds__(FSIG_BLOCK) k_join_multi(const MLevel* __restrict__ ML,
+                                                           const uint64_t* __restrict__ qbase, uint32_t nlive,
+                                                           const uint32_t* __restrict__ rows, uint64_t nrows,
+                                                           uint32_t p, uint32_t* __restrict__ cnt,
+                                                           const uint64_t* __restrict__ off,
+                                                           uint32_t* __restrict__ out,
+                                                           unsigned long long* __restrict__ qcounts,
+                                                           unsigned long long* __restrict__ stats) {
+  constexpr int GPB = FSIG_BLOCK / G;
+  constexpr int NR = (MAXQ + G - 1) / G;
+  __shared__ uint32_t sh_s[GPB][MAXQ];
+  __shared__ uint32_t sh_n[GPB][MAXQ];
+  __shared__ uint32_t sh_cf[GPB][MAXQ];
+  const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
+  const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
+  const uint64_t below = (1ull << lane) - 1;
+  const uint64_t ngroups = (uint64_t)gridDim.x * GPB;
+  uint64_t my_bytes = 0, my_units = 0;
+  uint32_t cur_q = 0xffffffffu;
+  unsigned long long acc = 0;
+  for (uint64_t r = (uint64_t)xcd_block() * GPB + gw; r < nrows; r += ngroups) {
+    uint32_t lo = 0, hi = nlive;  // query of the row: last j with qbase[j] <= r
+    while (hi - lo > 1) {
+      const uint32_t m = (lo + hi) >> 1;
+      if (qbase[m] <= r) lo = m; else hi = m;
+    }
+    const MLevel& M = ML[lo];
+    const JoinLevel& L = M.L;
+    const bool last = M.last != 0;
+    if (WRITE && last) continue;  // counted in the count pass
+    uint32_t rv[NR];
+    const uint32_t* row = rows + r * p;
+#pragma unroll
+    for (int j = 0; j < NR; j++) {
+      const uint32_t col = j * G + gl;
+      rv[j] = col < p ? row[col] : 0u;
+    }
+    uint32_t dlen = L.wsize, d = 0;
+    uint64_t elems = 0;
+    const uint32_t nb = L.nb, ncf = L.ncf;
+    for (uint32_t i = 0; i < nb; i++) {
+      const uint32_t x = group_col<G>(rv, L.bcol[i]);
+      const uint32_t* o = M.S.off[i];
+      const uint32_t s0 = o[x], n0 = o[x + 1] - s0;
+      sh_s[gw][i] = s0;
+      sh_n[gw][i] = n0;
+      elems += n0;
+      if (i == 0 || n0 < dlen) { dlen = n0; d = i; }
+    }
+    for (uint32_t j = 0; j < ncf; j++) {
+      uint32_t x = group_col<G>(rv, L.cfcol[j]);
+      if (!((L.cfsame >> j) & 1u)) x = L.cfcand[j][x];
+      sh_cf[gw][j] = x;
+    }
+    const uint32_t* dl = nb ? M.S.lst[d] + sh_s[gw][d] : nullptr;
+    uint64_t base = 0;
+    const bool direct = !WRITE && nb == 1;
+    if (direct) {
+      uint32_t hit = 0;
+      for (uint32_t j0 = 0; j0 < ncf; j0 += G) {
+        const uint32_t j = j0 + gl;
+        bool in = false;
+        if (j < ncf) {
+          uint32_t target = sh_cf[gw][j];
+          bool ok = true;
+          if (!((L.cfsame >> j) & 1u)) {
+            const uint32_t at = sorted_index(L.wcand, L.wsize, target);
+            ok = at < L.wsize && L.wcand[at] == target;
+            target = at;
+          }
+          in = ok && sorted_contains(dl, dlen, target);
+        }
+        hit += __popcll(__ballot(in) & gmask);
+      }
+      base = dlen - hit;
+      my_bytes += 4ull * ncf * (32 - __builtin_clz(dlen | 1));
+      elems = 0;
+    }
+    uint64_t child0 = WRITE ? off[r] : 0;
+    for (uint32_t t0 = 0; !direct && t0 < dlen; t0 += G) {
+      const uint32_t t = t0 + gl;
+      bool ok = t < dlen;
+      uint32_t x = 0;
+      if (ok) x = nb ? dl[t] : t;
+      for (uint32_t i = 0; i < nb && ok; i++) {
+        if (i == d) continue;
+        ok = sorted_contains(M.S.lst[i] + sh_s[gw][i], sh_n[gw][i], x);
+      }
+      if (ok && ncf) {
+        const uint32_t xid = L.any_cross ? L.wcand[x] : x;
+        for (uint32_t j = 0; j < ncf && ok; j++) ok = (((L.cfsame >> j) & 1u) ? x : xid) != sh_cf[gw][j];
+      }
+      const uint64_t bal = __ballot(ok) & gmask;
+      if (WRITE) {
+        uint32_t* dst = out + (child0 + base + __popcll(bal & below)) * (p + 1);
+        for (uint32_t c = 0; c < p; c++) {
+          const uint32_t v = group_col<G>(rv, c);
+          if (ok) dst[c] = v;
+        }
+        if (ok) dst[p] = x;
+      }
+      base += __popcll(bal);
+    }
+    if (gl == 0) {
+      my_bytes += 4ull * p + 8ull * nb + 4ull * elems + (WRITE ? 8 + base * 4ull * (p + 1) : 4);
+      my_units += WRITE ? base : 1;
+      if (!WRITE) {
+        if (last) {
+          if (M.qid != cur_q) {
+            if (cur_q != 0xffffffffu && acc) atomicAdd(&qcounts[cur_q], acc);
+            cur_q = M.qid;
+            acc = 0;
+          }
+          acc += base;
+          cnt[r] = 0;
+        } else {
+          cnt[r] = (uint32_t)base;
+        }
+      }
+    }
+  }
+  if (!WRITE && gl == 0 && cur_q != 0xffffffffu && acc) atomicAdd(&qcounts[cur_q], acc);
+  if (stats && gl == 0) {
+    atomicAdd(&stats[0], (unsigned long long)my_bytes);
+    atomicAdd(&stats[1], (unsigned long long)my_units);
+  }
+}
+
+__global__ void k_iota_multi(const uint64_t* __restrict__ qbase, uint32_t nlive, uint32_t* __restrict__ rows,
+                             uint64_t nrows) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = nlive;
+    while (hi - lo > 1) {
+      const uint32_t m = (lo + hi) >> 1;
+      if (qbase[m] <= r) lo = m; else hi = m;
+    }
+    rows[r] = (uint32_t)(r - qbase[lo]);
+  }
+}
+
+// off values at the query boundaries of a level: the next level's row bases.
+__global__ void k_gather_bases(const uint64_t* __restrict__ off, const uint64_t* __restrict__ qbase, uint32_t n,
+                               uint64_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= n) out[i] = off[qbase[i]];
+}
+
 __global__ void k_zero_u32(unsigned int* p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
+}
+
+// Resets the work queue for a run over `seeds` seed chunks (a kernel, so the
+// reset can live inside the captured graph).
+__global__ void k_queue_reset(Queue* qu, unsigned int* ready, uint32_t dcap, uint32_t seeds) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < dcap; i += gridDim.x * blockDim.x) ready[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    qu->seed_head = 0;
+    qu->don_head = 0;
+    qu->don_reserve = 0;
+    qu->outstanding = seeds;
+    qu->error = 0;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -455,26 +938,41 @@ struct TuneChoice {
   uint32_t G;
   bool lds;
   uint32_t tile;
+  bool lockstep;  // lockstep batched BFS instead of the persistent walk
+};
+
+struct DfsArgs {
+  const BQuery* Q;
+  const BLevel* LV;
+  const Task* T;
+  uint32_t nt;
+  Queue* qu;
+  DTask* dtasks;
+  unsigned int* ready;
+  uint32_t dcap;
+  uint32_t* stacks;
+  uint32_t stack_words;
+  unsigned long long* counts;
+  unsigned long long* stats;
 };
 
 template <int G, bool LDS>
-int launch_dfs_t(fsig_ctx* c, uint32_t grid, hipStream_t st, const BQuery* Q, const BLevel* LV, const Task* T,
-                 uint32_t nt, unsigned int* head, uint32_t* stacks, uint32_t sw, uint32_t tile,
-                 unsigned long long* counts, unsigned long long* stats) {
+int launch_dfs_t(fsig_ctx* c, uint32_t grid, hipStream_t st, const DfsArgs& a, uint32_t tile) {
   const size_t dyn = LDS ? (size_t)(FSIG_BLOCK / G) * tile * 4 : 0;
-  k_dfs<G, LDS><<<grid, FSIG_BLOCK, dyn, st>>>(Q, LV, T, nt, head, stacks, sw, tile, counts, stats);
+  k_dfs<G, LDS><<<grid, FSIG_BLOCK, dyn, st>>>(a.Q, a.LV, a.T, a.nt, a.qu, a.dtasks, a.ready, a.dcap, a.stacks,
+                                               a.stack_words, tile, a.counts, a.stats);
   FSIG_HIP(c, hipGetLastError());
   return FSIG_OK;
 }
 
-int launch_dfs(fsig_ctx* c, const TuneChoice& tc, uint32_t grid, hipStream_t st, const BQuery* Q, const BLevel* LV,
-               const Task* T, uint32_t nt, unsigned int* head, uint32_t* stacks, uint32_t sw,
-               unsigned long long* counts, unsigned long long* stats) {
+// Queue reset + persistent join for a run over the first `nt` seed chunks.
+int launch_dfs(fsig_ctx* c, const TuneChoice& tc, uint32_t grid, hipStream_t st, const DfsArgs& a) {
+  k_queue_reset<<<(a.dcap + 1023) / 1024, 1024, 0, st>>>(a.qu, a.ready, a.dcap, a.nt);
+  FSIG_HIP(c, hipGetLastError());
   const uint32_t tile = tc.lds ? tc.tile : 0;
-#define FSIG_DFS_CASE(GG)                                                                                   \
-  case GG:                                                                                                   \
-    return tc.lds ? launch_dfs_t<GG, true>(c, grid, st, Q, LV, T, nt, head, stacks, sw, tile, counts, stats) \
-                  : launch_dfs_t<GG, false>(c, grid, st, Q, LV, T, nt, head, stacks, sw, tile, counts, stats);
+#define FSIG_DFS_CASE(GG) \
+  case GG:                \
+    return tc.lds ? launch_dfs_t<GG, true>(c, grid, st, a, tile) : launch_dfs_t<GG, false>(c, grid, st, a, tile);
   switch (tc.G) {
     FSIG_DFS_CASE(4)
     FSIG_DFS_CASE(8)
@@ -503,6 +1001,23 @@ T* upload_table(fsig_ctx* c, const std::vector<T>& h) {
 
 }  // namespace
 
+// Stage timing on stderr when FSIG_DEBUG_TIMING is set (host clock, after a
+// stream synchronisation).
+struct StageClock {
+  bool on;
+  fsig_ctx* c;
+  std::chrono::steady_clock::time_point t0;
+  explicit StageClock(fsig_ctx* ctx) : on(getenv("FSIG_DEBUG_TIMING") != nullptr), c(ctx),
+                                       t0(std::chrono::steady_clock::now()) {}
+  void mark(const char* what) {
+    if (!on) return;
+    (void)hipStreamSynchronize(c->stream);
+    const auto t = std::chrono::steady_clock::now();
+    fprintf(stderr, "[fsig batch] %-24s %9.3f ms\n", what, std::chrono::duration<double, std::milli>(t - t0).count());
+    t0 = t;
+  }
+};
+
 // Per-context cache of tuning decisions, keyed by a coarse batch shape.
 static std::map<std::pair<const fsig_ctx*, uint64_t>, TuneChoice> g_tuned;
 
@@ -522,6 +1037,7 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
     ~Guard() { a.release(m); }
   } guard{A, mark};
   int rc;
+  StageClock clk(c);
   // ---- plans
   std::vector<QueryPlan> plans(nq);
   std::vector<uint8_t> active(nq, 1);
@@ -632,8 +1148,10 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
     for (size_t i = 0; i < tab.size(); i++) size[i] = (uint32_t)hs[i];
     return FSIG_OK;
   };
+  clk.mark("plans + LDF");
   if (ncls && (rc = run_compact(comp, tiles, 0, nullptr))) return rc;
   tf.done(4ull * n * nq + 16ull * flat, flat);
+  clk.mark("LDF compaction");
 
   // ---- refinement passes for every query
   std::vector<uint8_t> thr((size_t)std::max<uint32_t>(ncls, 1) * MAXQ, 0);
@@ -668,7 +1186,10 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
   kill_empty();
   const double avg_deg = n ? (double)g->nnz / n : 0;
   const uint32_t RG = avg_deg <= 12 ? 8 : (avg_deg <= 40 ? 16 : 64);
-  for (uint32_t pass = 0; any_target && pass < c->tune.refine_passes; pass++) {
+  // the batch caps refinement at two passes: later passes remove little and
+  // cost a full sweep each over every query's candidates
+  const uint32_t bpasses = std::min<uint32_t>(c->tune.refine_passes, 2);
+  for (uint32_t pass = 0; any_target && pass < bpasses; pass++) {
     KTimer tr(c, FSIG_K_REFINE);
     std::vector<BClass> cl;
     std::vector<BComp> rc_tab(ncls);
@@ -701,6 +1222,7 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
     if (!changed) break;
   }
   const std::vector<uint32_t*>& cand = *cur;
+  clk.mark("refinement");
 
   // ---- orders and the class pairs each query's join needs
   std::vector<BPair> pairs;
@@ -720,7 +1242,7 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
         if (pair_of[i][a * MAXQ + b] >= 0) continue;
         pair_of[i][a * MAXQ + b] = (int)pairs.size();
         const uint32_t ga = cls_base[i] + a, gb = cls_base[i] + b;
-        BPair pd{cand[ga], cand[gb], masks + (uint64_t)i * n, nullptr, size[ga], size[gb], b, 0, pflat};
+        BPair pd{cand[ga], cand[gb], masks + (uint64_t)i * n, nullptr, nullptr, size[ga], size[gb], b, 0, pflat};
         pd.off_out = (uint32_t*)A.alloc(4ull * (size[ga] + 1));
         if (!pd.off_out) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted for candidate edges");
         pflat += size[ga];
@@ -729,10 +1251,29 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
     }
   }
   const uint32_t np = (uint32_t)pairs.size();
-  std::vector<uint64_t> pbase_h(np), maxlen_h(np);
+  std::vector<uint64_t> pbase_h(np), maxlen_h(np), ptotal_h(np);
   uint32_t* lst_all = nullptr;
   if (np) {
     KTimer tc(c, FSIG_K_CEDGE);
+    // vertex -> candidate index maps of the target classes, n words each,
+    // while the arena holds a quarter of its free space for them
+    {
+      std::map<const uint32_t*, uint32_t*> made;
+      const uint64_t budget = A.avail() / 4;
+      uint64_t used = 0;
+      for (auto& pd : pairs) {
+        auto it = made.find(pd.bcand);
+        if (it == made.end()) {
+          uint32_t* m = nullptr;
+          if (used + 4ull * n <= budget && (m = (uint32_t*)A.alloc(4ull * n))) {
+            used += 4ull * n;
+            k_bscatter_pos<<<bgrid(c, pd.bsize, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(pd.bcand, pd.bsize, m);
+          }
+          it = made.emplace(pd.bcand, m).first;
+        }
+        pd.bpos = it->second;
+      }
+    }
     const BPair* dP = upload_table(c, pairs);
     uint32_t* cnt = (uint32_t*)A.alloc(4ull * std::max<uint64_t>(pflat, 1));
     uint64_t* gscan = (uint64_t*)A.alloc(8ull * (pflat + 1));
@@ -764,9 +1305,11 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
     FSIG_HIP(c, hipMemcpyAsync(mx.data(), dmax, 4ull * np, hipMemcpyDeviceToHost, c->stream));
     FSIG_HIP(c, hipStreamSynchronize(c->stream));
     for (uint32_t i = 0; i < np; i++) maxlen_h[i] = mx[i];
+    for (uint32_t i = 0; i < np; i++) ptotal_h[i] = (i + 1 < np ? pbase_h[i + 1] : total_edges) - pbase_h[i];
     tc.done(2ull * pflat * 12 + 12ull * total_edges, pflat);
   }
 
+  clk.mark("candidate edges");
   // ---- level tables, stacks and tasks
   std::vector<BQuery> bq(nq);
   std::vector<BLevel> lv;
@@ -775,6 +1318,10 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
   for (uint32_t i = 0; i < nq; i++) {
     bq[i].k = ks[i];
     bq[i].level_base = (uint32_t)lv.size();
+    bq[i].edge_tasks = 0;
+    bq[i].n1 = 0;
+    bq[i].off1 = nullptr;
+    bq[i].lst1 = nullptr;
     if (!active[i]) continue;
     const QueryPlan& P = plans[i];
     uint32_t soff = 0;
@@ -808,10 +1355,24 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
       lv.push_back(L);
     }
     stack_words = std::max(stack_words, soff);
-    // tasks: roots in chunks; smaller chunks for larger queries
+    // tasks: ranges of (root, first neighbour) list entries when position 1
+    // hangs off position 0 alone, else ranges of roots; smaller ranges for
+    // larger queries
     const uint32_t n1 = size[cls_base[i] + P.cls[P.order[0]]];
     const uint32_t chunk = P.k <= 4 ? 64 : (P.k <= 8 ? 16 : 4);
-    for (uint32_t r = 0; r < n1; r += chunk) tasks.push_back(Task{i, r, std::min(n1, r + chunk)});
+    const BLevel& l1 = lv[bq[i].level_base + 1 < lv.size() ? bq[i].level_base + 1 : bq[i].level_base];
+    if (P.k >= 2 && l1.nb == 1 && l1.bcol[0] == 0 && l1.ncf == 0) {
+      const int pi = pair_of[i][P.cls[P.order[0]] * MAXQ + P.cls[P.order[1]]];
+      const uint64_t e1 = ptotal_h[pi];
+      if (e1 >= 0xffffffffull) return set_err(c, FSIG_EUNSUPPORTED, "batch: first list too long");
+      bq[i].edge_tasks = 1;
+      bq[i].n1 = n1;
+      bq[i].off1 = pairs[pi].off_out;
+      bq[i].lst1 = lst_all + pbase_h[pi];
+      for (uint64_t r = 0; r < e1; r += chunk) tasks.push_back(Task{i, (uint32_t)r, (uint32_t)std::min<uint64_t>(e1, r + chunk)});
+    } else {
+      for (uint32_t r = 0; r < n1; r += chunk) tasks.push_back(Task{i, r, std::min(n1, r + chunk)});
+    }
   }
   if (tasks.empty()) {
     for (uint32_t i = 0; i < nq; i++) counts_out[i] = 0;
@@ -837,11 +1398,108 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
   const BLevel* dL = upload_table(c, lv);
   const Task* dT = upload_table(c, tasks);
   unsigned long long* dcounts = (unsigned long long*)A.alloc(8ull * nq);
-  unsigned int* head = (unsigned int*)A.alloc(64);
-  if (!dQ || !dL || !dT || !dcounts || !head) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted");
-  const uint32_t nt = (uint32_t)tasks.size();
+  Queue* qu = (Queue*)A.alloc(sizeof(Queue));
+  const uint32_t dcap = 1u << 20;
+  DTask* dtasks = (DTask*)A.alloc(sizeof(DTask) * (size_t)dcap);
+  unsigned int* ready = (unsigned int*)A.alloc(4ull * dcap);
+  if (!dQ || !dL || !dT || !dcounts || !qu || !dtasks || !ready)
+    return set_err(c, FSIG_ENOMEM, "batch: arena exhausted");
 
-  // ---- autotune over a sample of the queue, then the captured full run
+  clk.mark("level tables + tasks");
+  // ---- join engines
+  // Lockstep batched BFS over a set of live queries; counts land in dcounts.
+  const size_t join_mark = A.mark();
+  auto make_level = [&](uint32_t i, uint32_t p) {
+    MLevel M;
+    std::memset(&M, 0, sizeof(M));
+    M.qid = i;
+    const QueryPlan& P = plans[i];
+    if (p >= P.k) { M.last = 1; return M; }
+    const uint32_t w = P.order[p];
+    M.L.p = p;
+    M.L.wcand = cand[cls_base[i] + P.cls[w]];
+    M.L.wsize = size[cls_base[i] + P.cls[w]];
+    for (uint32_t cp = 0; cp < p; cp++) {
+      const uint32_t u = P.order[cp];
+      if ((P.adjm[w] >> u) & 1u) {
+        const int pi = pair_of[i][P.cls[u] * MAXQ + P.cls[w]];
+        M.L.bcol[M.L.nb] = cp;
+        M.S.off[M.L.nb] = pairs[pi].off_out;
+        M.S.lst[M.L.nb] = lst_all + pbase_h[pi];
+        M.L.nb++;
+      } else if (P.label[u] == P.label[w]) {
+        const uint32_t j = M.L.ncf++;
+        M.L.cfcol[j] = cp;
+        if (P.cls[u] == P.cls[w]) M.L.cfsame |= 1u << j;
+        else { M.L.cfcand[j] = cand[cls_base[i] + P.cls[u]]; M.L.any_cross = 1; }
+      }
+    }
+    M.last = p + 1 == P.k;
+    return M;
+  };
+  unsigned long long* stats_ls = family_counters(c, FSIG_K_BATCH);
+  auto lockstep = [&](uint32_t G, const std::vector<uint32_t>& qs, float& ms) -> int {
+    const size_t mk = A.mark();
+    std::vector<uint32_t> live;
+    uint32_t maxk = 0;
+    for (uint32_t i : qs)
+      if (active[i] && ks[i] >= 2) { live.push_back(i); maxk = std::max(maxk, ks[i]); }
+    const uint32_t nl = (uint32_t)live.size();
+    FSIG_HIP(c, hipEventRecord(c->ev_a, c->stream));
+    if (nl) {
+      std::vector<uint64_t> qb(nl + 1, 0);
+      for (uint32_t j = 0; j < nl; j++)
+        qb[j + 1] = qb[j] + size[cls_base[live[j]] + plans[live[j]].cls[plans[live[j]].order[0]]];
+      uint64_t* dqb = (uint64_t*)A.alloc(8ull * (nl + 1));
+      uint64_t* dqb2 = (uint64_t*)A.alloc(8ull * (nl + 1));
+      MLevel* dml = (MLevel*)A.alloc(sizeof(MLevel) * nl);
+      uint32_t* rows = (uint32_t*)A.alloc(4ull * std::max<uint64_t>(qb[nl], 1));
+      if (!dqb || !dqb2 || !dml || !rows) { A.release(mk); return set_err(c, FSIG_ENOMEM, "lockstep: arena exhausted"); }
+      FSIG_HIP(c, hipMemcpyAsync(dqb, qb.data(), 8ull * (nl + 1), hipMemcpyHostToDevice, c->stream));
+      uint64_t R = qb[nl];
+      k_iota_multi<<<bgrid(c, R, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(dqb, nl, rows, R);
+      std::vector<MLevel> ml(nl);
+      for (uint32_t p = 1; p < maxk && R > 0; p++) {
+        bool any_more = false;
+        for (uint32_t j = 0; j < nl; j++) {
+          ml[j] = make_level(live[j], p);
+          any_more |= ml[j].last == 0;
+        }
+        FSIG_HIP(c, hipMemcpyAsync(dml, ml.data(), sizeof(MLevel) * nl, hipMemcpyHostToDevice, c->stream));
+        uint32_t* cnt = (uint32_t*)A.alloc(4ull * R);
+        uint64_t* off = (uint64_t*)A.alloc(8ull * (R + 1));
+        if (!cnt || !off) { A.release(mk); return set_err(c, FSIG_ENOMEM, "lockstep: arena exhausted"); }
+        const uint32_t grid = bgrid(c, R, FSIG_BLOCK / G);
+#define FSIG_LS(GG, W) k_join_multi<GG, W><<<grid, FSIG_BLOCK, 0, c->stream>>>(dml, dqb, nl, rows, R, p, cnt, off, out, dcounts, stats_ls)
+        uint32_t* out = nullptr;
+        switch (G) { case 4: FSIG_LS(4, false); break; case 8: FSIG_LS(8, false); break;
+                     case 16: FSIG_LS(16, false); break; default: FSIG_LS(64, false); break; }
+        FSIG_HIP(c, hipGetLastError());
+        if (!any_more) break;
+        int rc2 = scan_u32_to_u64(c, cnt, off, R);
+        if (rc2) { A.release(mk); return rc2; }
+        uint64_t T;
+        if ((rc2 = read_u64(c, off + R, &T, 1))) { A.release(mk); return rc2; }
+        if (T == 0) break;
+        out = (uint32_t*)A.alloc(4ull * (p + 1) * T);
+        if (!out) { A.release(mk); return set_err(c, FSIG_ENOMEM, "lockstep: a level does not fit the arena"); }
+        switch (G) { case 4: FSIG_LS(4, true); break; case 8: FSIG_LS(8, true); break;
+                     case 16: FSIG_LS(16, true); break; default: FSIG_LS(64, true); break; }
+#undef FSIG_LS
+        k_gather_bases<<<(nl + 256) / 256, 256, 0, c->stream>>>(off, dqb, nl, dqb2);
+        FSIG_HIP(c, hipGetLastError());
+        std::swap(dqb, dqb2);
+        rows = out;
+        R = T;
+      }
+    }
+    FSIG_HIP(c, hipEventRecord(c->ev_b, c->stream));
+    FSIG_HIP(c, hipEventSynchronize(c->ev_b));
+    FSIG_HIP(c, hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
+    A.release(mk);
+    return FSIG_OK;
+  };
+  // Persistent work-queue walk over the seed tasks of a set of queries.
   auto groups_for = [&](uint32_t G, uint32_t grid) { return (uint64_t)grid * (FSIG_BLOCK / G); };
   auto grid_for_G = [&](uint32_t G) {
     uint64_t grid = (uint64_t)c->num_cus * 8;
@@ -849,71 +1507,109 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
     while (grid > 64 && groups_for(G, (uint32_t)grid) * stack_words * 4ull > avail) grid >>= 1;
     return (uint32_t)grid;
   };
-  const uint32_t max_grid = grid_for_G(4);
-  uint32_t* stacks = (uint32_t*)A.alloc(groups_for(4, max_grid) * stack_words * 4ull);
-  if (!stacks) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted for DFS stacks");
+  auto persistent = [&](const TuneChoice& tc, const std::vector<uint32_t>& qs, bool capture, float& ms,
+                        unsigned long long* stats) -> int {
+    const size_t mk = A.mark();
+    std::vector<uint8_t> in(nq, 0);
+    for (uint32_t i : qs) in[i] = 1;
+    std::vector<Task> sub;
+    for (const Task& t : tasks)
+      if (in[t.q]) sub.push_back(t);
+    const Task* dsub = upload_table(c, sub);
+    const uint32_t grid = grid_for_G(tc.G);
+    uint32_t* stacks = (uint32_t*)A.alloc(groups_for(tc.G, grid) * stack_words * 4ull);
+    if (!dsub || !stacks) { A.release(mk); return set_err(c, FSIG_ENOMEM, "batch: arena exhausted for DFS stacks"); }
+    const DfsArgs da{dQ, dL, dsub, (uint32_t)sub.size(), qu, dtasks, ready, dcap, stacks, stack_words, dcounts, stats};
+    int rc2;
+    if (capture) {
+      // queue reset + persistent kernel, captured once and launched as a graph
+      hipGraph_t graph;
+      hipGraphExec_t exec;
+      FSIG_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      rc2 = launch_dfs(c, tc, grid, c->stream, da);
+      hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+      if (rc2) { A.release(mk); return rc2; }
+      if (ce != hipSuccess) { A.release(mk); return set_err(c, FSIG_EHIP, std::string("batch: capture: ") + hipGetErrorString(ce)); }
+      FSIG_HIP(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      FSIG_HIP(c, hipEventRecord(c->ev_a, c->stream));
+      FSIG_HIP(c, hipGraphLaunch(exec, c->stream));
+      FSIG_HIP(c, hipEventRecord(c->ev_b, c->stream));
+      FSIG_HIP(c, hipEventSynchronize(c->ev_b));
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+    } else {
+      FSIG_HIP(c, hipEventRecord(c->ev_a, c->stream));
+      if ((rc2 = launch_dfs(c, tc, grid, c->stream, da))) { A.release(mk); return rc2; }
+      FSIG_HIP(c, hipEventRecord(c->ev_b, c->stream));
+      FSIG_HIP(c, hipEventSynchronize(c->ev_b));
+    }
+    FSIG_HIP(c, hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
+    uint32_t qerr = 0;
+    FSIG_HIP(c, hipMemcpyAsync(c->h_pinned, &qu->error, 4, hipMemcpyDeviceToHost, c->stream));
+    FSIG_HIP(c, hipStreamSynchronize(c->stream));
+    qerr = *(const uint32_t*)c->h_pinned;
+    A.release(mk);
+    if (qerr) return set_err(c, FSIG_EHIP, "batch: a work-queue wait ran past its bound");
+    return FSIG_OK;
+  };
+  std::vector<uint32_t> all_q;
+  for (uint32_t i = 0; i < nq; i++)
+    if (active[i]) all_q.push_back(i);
   const uint64_t shape = ((uint64_t)nq << 40) ^ ((uint64_t)g->n << 8) ^ (uint64_t)stack_words;
-  TuneChoice best{16, false, 0};
+  TuneChoice best{16, false, 0, true};
   if (c->tune.lanes_per_row) {
-    best = TuneChoice{c->tune.lanes_per_row, c->tune.strategy == 2, c->tune.lds_tile ? c->tune.lds_tile : 256};
+    best = TuneChoice{c->tune.lanes_per_row, c->tune.strategy == 2, c->tune.lds_tile ? c->tune.lds_tile : 256,
+                      c->tune.strategy != 1 && c->tune.strategy != 2};
   } else {
     auto it = g_tuned.find({c, shape});
     if (it != g_tuned.end()) {
       best = it->second;
     } else {
-      // sample: a prefix of the interleaved queue (every query represented)
-      const uint32_t sample = std::min<uint32_t>(nt, std::max<uint32_t>(nq * 2, nt / 16));
-      const TuneChoice cands[] = {{4, false, 0}, {8, false, 0}, {16, false, 0}, {32, false, 0},
-                                  {8, true, 128}, {16, true, 256}, {32, true, 512}};
+      // sample: every 8th live query; both engines, several shapes each
+      std::vector<uint32_t> sample;
+      for (size_t j = 0; j < all_q.size(); j += 8) sample.push_back(all_q[j]);
+      const TuneChoice cands[] = {{4, false, 0, true},   {8, false, 0, true},   {16, false, 0, true},
+                                  {64, false, 0, true},  {16, false, 0, false}, {64, false, 0, false},
+                                  {16, true, 256, false}};
       float best_ms = 1e30f;
       for (const TuneChoice& tc : cands) {
-        const uint32_t grid = std::min(max_grid, grid_for_G(tc.G));
-        k_zero_u32<<<1, 64, 0, c->stream>>>(head, 1);
-        FSIG_HIP(c, hipMemsetAsync(dcounts, 0, 8ull * nq, c->stream));
-        FSIG_HIP(c, hipEventRecord(c->ev_a, c->stream));
-        if ((rc = launch_dfs(c, tc, grid, c->stream, dQ, dL, dT, sample, head, stacks, stack_words, dcounts, nullptr)))
-          return rc;
-        FSIG_HIP(c, hipEventRecord(c->ev_b, c->stream));
-        FSIG_HIP(c, hipEventSynchronize(c->ev_b));
         float ms = 0;
-        FSIG_HIP(c, hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
+        FSIG_HIP(c, hipMemsetAsync(dcounts, 0, 8ull * nq, c->stream));
+        rc = tc.lockstep ? lockstep(tc.G, sample, ms) : persistent(tc, sample, false, ms, nullptr);
+        if (rc == FSIG_ENOMEM) continue;  // this engine cannot hold the sample
+        if (rc) return rc;
         if (ms < best_ms) { best_ms = ms; best = tc; }
       }
       g_tuned[{c, shape}] = best;
     }
   }
-  const uint32_t grid = std::min(max_grid, grid_for_G(best.G));
-  // the full run: queue reset + persistent kernel, captured once as a graph
+  clk.mark("autotune");
   FSIG_HIP(c, hipMemsetAsync(dcounts, 0, 8ull * nq, c->stream));
-  unsigned long long* stats = family_counters(c, FSIG_K_BATCH);
-  hipGraph_t graph;
-  hipGraphExec_t exec;
-  FSIG_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  k_zero_u32<<<1, 64, 0, c->stream>>>(head, 1);
-  rc = launch_dfs(c, best, grid, c->stream, dQ, dL, dT, nt, head, stacks, stack_words, dcounts, stats);
-  hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+  float ms = 0;
+  {
+    KTimer tj(c, FSIG_K_BATCH);
+    rc = best.lockstep ? lockstep(best.G, all_q, ms) : persistent(best, all_q, true, ms, family_counters(c, FSIG_K_BATCH));
+    tj.done(0, 0);
+  }
+  if (rc == FSIG_ENOMEM && best.lockstep) {  // a level did not fit: the walk needs no tables
+    FSIG_HIP(c, hipMemsetAsync(dcounts, 0, 8ull * nq, c->stream));
+    best = TuneChoice{64, false, 0, false};
+    rc = persistent(best, all_q, true, ms, family_counters(c, FSIG_K_BATCH));
+  }
   if (rc) return rc;
-  if (ce != hipSuccess) return set_err(c, FSIG_EHIP, std::string("batch: capture: ") + hipGetErrorString(ce));
-  FSIG_HIP(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  KTimer tj(c, FSIG_K_BATCH);
-  FSIG_HIP(c, hipEventRecord(c->ev_a, c->stream));
-  FSIG_HIP(c, hipGraphLaunch(exec, c->stream));
-  FSIG_HIP(c, hipEventRecord(c->ev_b, c->stream));
-  tj.done(0, 0);
+  (void)join_mark;
   std::vector<uint64_t> hc(nq);
   for (uint32_t i = 0; i < nq; i += kPinnedWords)
     if ((rc = read_u64(c, dcounts + i, hc.data() + i, std::min<size_t>(kPinnedWords, nq - i)))) return rc;
-  float ms = 0;
-  FSIG_HIP(c, hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
-  (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
   for (uint32_t i = 0; i < nq; i++) {
     counts_out[i] = active[i] ? hc[i] : 0;
     if (active[i] && ks[i] == 1) counts_out[i] = size[cls_base[i] + plans[i].cls[0]];
   }
+  clk.mark("join");
   if (join_ms) *join_ms = ms;
   c->last_tune_G = best.G;
   c->last_tune_lds = best.lds ? best.tile : 0;
+  c->last_tune_engine = best.lockstep ? 1 : 2;
   c->last_peak = A.peak - mark;
   FSIG_HIP(c, hipStreamSynchronize(c->stream));
   return resolve_timings(c);

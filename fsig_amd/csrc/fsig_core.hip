@@ -366,6 +366,12 @@ int fsig_stream_copy(fsig_ctx* c, uint64_t bytes, uint32_t iters, double* gbps) 
   return FSIG_OK;
 }
 
+int fsig_set_work_limit(fsig_ctx* c, uint64_t max_rows) {
+  if (!c) return FSIG_EINVAL;
+  c->work_limit = max_rows;
+  return FSIG_OK;
+}
+
 int fsig_set_profiling(fsig_ctx* c, int on) {
   if (!c) return FSIG_EINVAL;
   c->profiling = on != 0;
@@ -396,10 +402,11 @@ int fsig_last_candidates(const fsig_ctx* c, uint32_t k, uint32_t* sizes) {
 
 uint64_t fsig_last_peak_bytes(const fsig_ctx* c) { return c ? c->last_peak : 0; }
 
-int fsig_last_batch_tune(const fsig_ctx* c, uint32_t* lanes, uint32_t* tile) {
+int fsig_last_batch_tune(const fsig_ctx* c, uint32_t* lanes, uint32_t* tile, uint32_t* engine) {
   if (!c) return FSIG_EINVAL;
   if (lanes) *lanes = c->last_tune_G;
   if (tile) *tile = c->last_tune_lds;
+  if (engine) *engine = c->last_tune_engine;
   return FSIG_OK;
 }
 

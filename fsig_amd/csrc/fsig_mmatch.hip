@@ -75,6 +75,7 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_refine(RefineTable T, const uint
                                                        const uint32_t* __restrict__ mask,
                                                        uint8_t* __restrict__ keep) {
   constexpr int GPB = FSIG_BLOCK / G;
+  __shared__ uint8_t rem[GPB][MAXQ];  // outstanding need per target class
   const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
   const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
   const uint32_t total = T.base[T.n];
@@ -84,21 +85,23 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_refine(RefineTable T, const uint
     while (a + 1 < T.n && T.base[a + 1] <= f) a++;
     const uint32_t v = T.cand[a][f - T.base[a]];
     const uint32_t s = off[v], e = off[v + 1];
-    bool ok = true;
-    uint32_t tm = T.tmask[a];
-    while (tm && ok) {
-      const uint32_t b = __builtin_ctz(tm);
-      tm &= tm - 1;
-      const uint32_t need = T.thr[a][b];
-      uint32_t got = 0;
-      for (uint32_t j = s; j < e && got < need; j += G) {
-        const uint32_t i = j + gl;
-        const bool hit = i < e && ((mask[adj[i]] >> b) & 1u);
-        got += __popcll(__ballot(hit) & gmask);
+    // one pass over the adjacency: each gathered mask word serves every
+    // target class; stop once all thresholds are met
+    uint32_t open = T.tmask[a];
+    for (uint32_t tm = open; tm; tm &= tm - 1) rem[gw][__builtin_ctz(tm)] = T.thr[a][__builtin_ctz(tm)];
+    for (uint32_t j = s; j < e && open; j += G) {
+      const uint32_t i = j + gl;
+      const uint32_t m = i < e ? (mask[adj[i]] & open) : 0u;
+      for (uint32_t tm = open; tm; tm &= tm - 1) {
+        const uint32_t b = __builtin_ctz(tm);
+        const uint32_t got = __popcll(__ballot((m >> b) & 1u) & gmask);
+        const uint32_t r = rem[gw][b];
+        const uint32_t nr = r > got ? r - got : 0;
+        rem[gw][b] = (uint8_t)nr;
+        if (nr == 0) open &= ~(1u << b);
       }
-      ok = got >= need;
     }
-    if (gl == 0) keep[f] = ok ? 1 : 0;
+    if (gl == 0) keep[f] = open == 0 ? 1 : 0;
   }
 }
 
@@ -127,8 +130,14 @@ struct PairDesc {
   const uint32_t* acand;
   const uint32_t* bcand;
   uint32_t* off_out;
+  const uint32_t* bpos;  // vertex -> candidate index of b (null: binary search)
   uint32_t asize, bsize, bbit, flat_base;
 };
+
+// bpos[C[j]] = j: the vertex -> candidate index map of one class.
+__global__ void k_scatter_pos(const uint32_t* __restrict__ cand, uint32_t n, uint32_t* __restrict__ pos) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) pos[cand[j]] = j;
+}
 
 __device__ __forceinline__ uint32_t find_pair(const PairDesc* P, uint32_t np, uint64_t f) {
   uint32_t lo = 0, hi = np;  // last pair with flat_base <= f
@@ -195,7 +204,7 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_cedge_fill(const PairDesc* __res
         hit = (mask[x] >> pd.bbit) & 1u;
       }
       const uint64_t bal = __ballot(hit) & gmask;
-      if (hit) lst_all[at + __popcll(bal & below)] = sorted_index(pd.bcand, pd.bsize, x);
+      if (hit) lst_all[at + __popcll(bal & below)] = pd.bpos ? pd.bpos[x] : sorted_index(pd.bcand, pd.bsize, x);
       at += __popcll(bal);
     }
   }
@@ -416,6 +425,7 @@ int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_
       if (pair_id[a][b] >= 0) continue;
       pair_id[a][b] = (int)pairs.size();
       PairDesc pd;
+      pd.bpos = nullptr;
       pd.acand = cur[a];
       pd.bcand = cur[b];
       pd.asize = size[a];
@@ -442,6 +452,23 @@ int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_
       if (!pd.off_out) return set_err(c, FSIG_ENOMEM, "cedge: arena exhausted");
     }
     if (flat >= 0xffffffffull) return set_err(c, FSIG_EUNSUPPORTED, "cedge: too many candidate rows");
+    // vertex -> candidate index maps of the target classes (n words each),
+    // when the arena holds them; otherwise the fill binary-searches
+    {
+      uint32_t* bpos_of[MAXQ] = {nullptr};
+      bool fit = true;
+      for (auto& pd : pairs) {
+        const uint32_t b = pd.bbit;
+        if (!bpos_of[b] && fit) {
+          bpos_of[b] = (uint32_t*)A.alloc(4ull * g->n);
+          if (!bpos_of[b]) { fit = false; continue; }
+          k_scatter_pos<<<grid_for(c, pd.bsize, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(pd.bcand, pd.bsize, bpos_of[b]);
+        }
+        pd.bpos = fit ? bpos_of[b] : nullptr;
+      }
+      if (!fit)
+        for (auto& pd : pairs) pd.bpos = nullptr;
+    }
     PairDesc* dP = (PairDesc*)A.alloc(sizeof(PairDesc) * np);
     uint32_t* cnt = (uint32_t*)A.alloc(4ull * flat);
     uint64_t* gscan = (uint64_t*)A.alloc(8ull * (flat + 1));

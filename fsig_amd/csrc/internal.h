@@ -74,8 +74,10 @@ struct fsig_ctx {
   unsigned long long* h_pinned = nullptr;  // kPinnedWords words of pinned staging
   int num_cus = 256;
   // choice of the batched engine's autotuner on its last batch
+  uint64_t work_limit = 0;  // max intermediate rows per fsig_match (0 = none)
   uint32_t last_tune_G = 0;
   uint32_t last_tune_lds = 0;
+  uint32_t last_tune_engine = 0;  // 1 lockstep batched BFS, 2 persistent walk
 };
 
 struct UnitMeta {
@@ -146,7 +148,7 @@ int set_err(fsig_ctx* c, int code, const std::string& msg);
 struct KTimer {
   fsig_ctx* c;
   int fam;
-  TimedLaunch* tl = nullptr;
+  size_t slot = ~(size_t)0;  // index, not pointer: nested timers may grow the vector
   explicit KTimer(fsig_ctx* ctx, int family) : c(ctx), fam(family) {
     if (c->profiling) {
       if (c->timed_used == c->timed.size()) {
@@ -155,9 +157,9 @@ struct KTimer {
         (void)hipEventCreate(&t.b);
         c->timed.push_back(t);
       }
-      tl = &c->timed[c->timed_used++];
-      tl->fam = family;
-      (void)hipEventRecord(tl->a, c->stream);
+      slot = c->timed_used++;
+      c->timed[slot].fam = family;
+      (void)hipEventRecord(c->timed[slot].a, c->stream);
     }
   }
   // alg_bytes / units: host-known figures (kernels that account on the
@@ -167,7 +169,7 @@ struct KTimer {
     s.launches++;
     s.alg_bytes += alg_bytes;
     s.units += units;
-    if (tl) (void)hipEventRecord(tl->b, c->stream);
+    if (slot != ~(size_t)0) (void)hipEventRecord(c->timed[slot].b, c->stream);
   }
 };
 // Adds the recorded event intervals and the device-side family counters into

@@ -58,6 +58,7 @@ struct JoinIO {
 // the backward neighbour's class) is lst[i][off[i][x] .. off[i][x+1]), holding
 // candidate indices of w's class.
 struct CandEdgeSrc {
+  static constexpr bool kAllMembers = true;  // lists hold only candidates of w
   const uint32_t* off[MAXQ];
   const uint32_t* lst[MAXQ];
   __device__ __forceinline__ void range(uint32_t i, uint32_t x, uint32_t& s, uint32_t& n) const {
@@ -74,6 +75,7 @@ struct CandEdgeSrc {
 // (the lookup of match/GSI_kernels.cuh:95-133 with a correct chain walk).
 // Elements are data vertex ids; membership in C(w) is a bit of `mask`.
 struct PcsrSrc {
+  static constexpr bool kAllMembers = false;  // lists hold every neighbour of the partner label
   const uint32_t* groups;
   const uint32_t* edges;
   const uint32_t* mask;
@@ -161,7 +163,31 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join(JoinLevel L, Src S, JoinIO 
     uint64_t child0 = 0;
     if (WRITE) child0 = io.off[r];
     const uint32_t* dl = L.nb ? S.list(d) + sh_s[gw][d] : nullptr;
-    for (uint32_t t0 = 0; t0 < dlen; t0 += G) {
+    // Count pass with one backward list of candidates only: the children are
+    // the list minus the earlier vertices of the same label that occur in
+    // it, found by binary search (one lane per conflict).
+    const bool direct = !WRITE && Src::kAllMembers && L.nb == 1;
+    if (direct) {
+      uint32_t hit = 0;
+      for (uint32_t j0 = 0; j0 < L.ncf; j0 += G) {
+        const uint32_t j = j0 + gl;
+        bool in = false;
+        if (j < L.ncf) {
+          uint32_t target = sh_cf[gw][j];
+          bool ok = true;
+          if (!((L.cfsame >> j) & 1u)) {  // data vertex id -> candidate index of w
+            const uint32_t at = sorted_index(L.wcand, L.wsize, target);
+            ok = at < L.wsize && L.wcand[at] == target;
+            target = at;
+          }
+          in = ok && sorted_contains(dl, dlen, target);
+        }
+        hit += __popcll(__ballot(in) & gmask);
+      }
+      base = dlen - hit;
+      elems = 0;  // the list itself is not read
+    }
+    for (uint32_t t0 = 0; !direct && t0 < dlen; t0 += G) {
       const uint32_t t = t0 + gl;
       bool ok = t < dlen;
       uint32_t x = 0;
@@ -194,9 +220,12 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join(JoinLevel L, Src S, JoinIO 
     if (!WRITE && io.cnt && gl == 0) io.cnt[r] = (uint32_t)base;
     if (gl == 0) {
       // algorithmic bytes: row words, an offset pair per backward neighbour,
-      // each list element once; then a u32 count (count pass) or a u64
-      // offset and 4(p+1) bytes per child written (write pass)
+      // each list element read once (none on the direct count path, where
+      // only the probes of the binary searches are charged); then a u32
+      // count (count pass) or a u64 offset and 4(p+1) bytes per child
+      // written (write pass)
       uint64_t b = 4ull * L.p + 8ull * L.nb + 4ull * elems;
+      if (direct) b += 4ull * L.ncf * (32 - __builtin_clz(dlen | 1));
       if (WRITE) {
         const uint64_t lo = child0 > io.out_lo ? child0 : io.out_lo;
         const uint64_t hi = (child0 + base) < io.out_hi ? child0 + base : io.out_hi;
@@ -223,30 +252,24 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join(JoinLevel L, Src S, JoinIO 
   }
 }
 
-// Finds, for child windows [w*cap, (w+1)*cap), the first and one-past-last
-// parent rows: lo = last r with off[r] <= w*cap, hi = first r with off[r] >=
-// min((w+1)*cap, total).  off has nrows+1 entries.
-static __global__ void k_window_rows(const uint64_t* __restrict__ off, uint64_t nrows, uint64_t cap,
-                                     uint64_t total, uint32_t nwin, unsigned long long* __restrict__ out) {
-  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= nwin) return;
-  const uint64_t lo = (uint64_t)w * cap;
-  const uint64_t hi = lo + cap < total ? lo + cap : total;
-  // first row: last r with off[r] <= lo, i.e. upper_bound(lo) - 1
+// Parent rows of the child window [lo, hi): out[0] = last r with off[r] <= lo,
+// out[1] = first r with off[r] >= hi (off has nrows+1 entries).
+static __global__ void k_window_rows(const uint64_t* __restrict__ off, uint64_t nrows, uint64_t lo, uint64_t hi,
+                                     unsigned long long* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
   uint64_t a = 0, b = nrows + 1;
   while (a < b) {
-    uint64_t m = (a + b) >> 1;
+    const uint64_t m = (a + b) >> 1;
     if (off[m] <= lo) a = m + 1; else b = m;
   }
-  out[2 * w] = a - 1;
-  // one past the last row: first r with off[r] >= hi
+  out[0] = a - 1;
   a = 0;
   b = nrows + 1;
   while (a < b) {
-    uint64_t m = (a + b) >> 1;
+    const uint64_t m = (a + b) >> 1;
     if (off[m] < hi) a = m + 1; else b = m;
   }
-  out[2 * w + 1] = a > nrows ? nrows : a;
+  out[1] = a > nrows ? nrows : a;
 }
 
 // Host launcher for the join kernel, dispatching on G.
@@ -291,7 +314,7 @@ class JoinDriver {
   int run(const uint32_t* level1, uint64_t n1) { return expand(1, level1, n1); }
 
  private:
-  bool bounded() const { return mode_ == FSIG_MODE_ENUMERATE && max_emb_ != 0; }
+  bool bounded() const { return max_emb_ != 0; }
   uint64_t remaining() const { return bounded() ? max_emb_ - total_ : ~0ull; }
 
   int expand(uint32_t p, const uint32_t* rows, uint64_t R) {
@@ -319,7 +342,7 @@ class JoinDriver {
       t.done(0, 0);
       uint64_t h;
       if ((rc = read_u64(c_, res, &h, 1))) return rc;
-      total_ += h;
+      total_ += std::min(h, remaining());
       return FSIG_OK;
     }
     uint32_t* cnt = (uint32_t*)c_->arena.alloc(4 * R);
@@ -336,32 +359,32 @@ class JoinDriver {
     if ((rc = read_u64(c_, off + R, &T, 1))) return rc;
     if (T == 0) { c_->arena.release(mk); return FSIG_OK; }
     if (last) T = std::min(T, remaining());
+    const uint64_t limit = c_->work_limit;
+    if (limit && !last && rows_ + T > limit)
+      return set_err(c_, FSIG_ELIMIT, "join: work limit exceeded");
+    rows_ += last ? 0 : T;
     // Children per window: half of the free arena, each child costing its
     // row plus the count/offset words it needs when expanded in turn.
     const uint64_t per_child = 4ull * (p + 1) + 16;
     uint64_t cap = c_->arena.avail() / 2 / per_child;
     if (cap == 0) return set_err(c_, FSIG_ENOMEM, "join: arena exhausted for child rows");
-    uint64_t nwin = (T + cap - 1) / cap;
-    if (nwin > 2048) return set_err(c_, FSIG_ENOMEM, "join: level needs more than 2048 windows");
+    const uint64_t nwin = (T + cap - 1) / cap;
     const uint64_t wcap = (T + nwin - 1) / nwin;  // balanced windows
-    std::vector<uint64_t> win(2 * nwin);
-    if (nwin == 1) {
-      win[0] = 0;
-      win[1] = R;
-    } else {
-      unsigned long long* dwin = (unsigned long long*)c_->arena.alloc(16 * nwin);
-      if (!dwin) return set_err(c_, FSIG_ENOMEM, "join: arena exhausted for windows");
-      k_window_rows<<<(uint32_t)((nwin + 255) / 256), 256, 0, c_->stream>>>(off, R, wcap, T, (uint32_t)nwin, dwin);
-      FSIG_HIP(c_, hipGetLastError());
-      if ((rc = read_u64(c_, dwin, win.data(), 2 * nwin))) return rc;
-    }
+    unsigned long long* dwin = nwin > 1 ? (unsigned long long*)c_->arena.alloc(16) : nullptr;
+    if (nwin > 1 && !dwin) return set_err(c_, FSIG_ENOMEM, "join: arena exhausted for windows");
     for (uint64_t w = 0; w < nwin && remaining() > 0; w++) {
       const uint64_t lo = w * wcap, hi = std::min(T, lo + wcap);
       if (lo >= hi) break;
+      uint64_t win[2] = {0, R};
+      if (nwin > 1) {  // parent rows of this window, found on the device
+        k_window_rows<<<1, 64, 0, c_->stream>>>(off, R, lo, hi, dwin);
+        FSIG_HIP(c_, hipGetLastError());
+        if ((rc = read_u64(c_, dwin, win, 2))) return rc;
+      }
       const size_t m2 = c_->arena.mark();
       uint32_t* child = (uint32_t*)c_->arena.alloc(4ull * (p + 1) * (hi - lo));
       if (!child) return set_err(c_, FSIG_ENOMEM, "join: arena exhausted for a window");
-      JoinIO io{rows, win[2 * w], win[2 * w + 1], nullptr, off, child, lo, hi, nullptr,
+      JoinIO io{rows, win[0], win[1], nullptr, off, child, lo, hi, nullptr,
                 family_counters(c_, FSIG_K_JOIN_WRITE)};
       {
         KTimer t(c_, FSIG_K_JOIN_WRITE);
@@ -382,6 +405,7 @@ class JoinDriver {
   uint64_t max_emb_;
   LeafSink* sink_;
   uint64_t total_ = 0;
+  uint64_t rows_ = 0;  // intermediate rows produced so far
 };
 
 // Lanes per row: the tuned value, or the power of two at or above the

@@ -72,10 +72,15 @@ def c4_graph() -> HostGraph:
 
 
 def walk_queries(g: HostGraph, count: int, kmin: int, kmax: int, seed: int, log_uniform: bool = False,
-                 max_edges: int = 10 ** 6) -> List[Tuple[str, Query]]:
+                 max_edges: int = 10 ** 6, dg=None, max_count: int = 0,
+                 max_rows: int = 0) -> List[Tuple[str, Query]]:
     """count queries, k uniform (or log-uniform) in [kmin, kmax], alternating
     dense (average degree >= 3) and sparse; seeds that find no walk with the
-    wanted density are skipped deterministically."""
+    wanted density are skipped deterministically.  With a device graph `dg`,
+    a query is also rejected when it has more than max_count embeddings or
+    its plain join would build more than max_rows intermediate rows (the
+    engine's bounded count and work limit decide, so the check stops early
+    on huge queries)."""
     rng = np.random.default_rng(seed)
     out = []
     s = seed * 100003
@@ -91,6 +96,19 @@ def walk_queries(g: HostGraph, count: int, kmin: int, kmax: int, seed: int, log_
             q = g.walk_query(k, lo, hi, seed=s, bias=2 if dense else 0, max_attempts=2000)
         except FsigError:
             continue
+        if dg is not None and (max_count or max_rows):
+            from . import match, ALG_MMATCH, MODE_COUNT, FSIG_ELIMIT
+            dg.ctx.set_work_limit(max_rows)
+            try:
+                c = match(dg, q, ALG_MMATCH, MODE_COUNT, max_embeddings=max_count + 1 if max_count else 0).count
+            except FsigError as e:
+                if e.code != FSIG_ELIMIT:
+                    raise
+                continue
+            finally:
+                dg.ctx.set_work_limit(0)
+            if max_count and c > max_count:
+                continue
         out.append(("dense" if dense else "sparse", q))
     return out
 

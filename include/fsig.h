@@ -33,7 +33,8 @@ enum {
   FSIG_EHIP = -3,         /* HIP runtime error                             */
   FSIG_EUNSUPPORTED = -4, /* e.g. more than 32 query vertices              */
   FSIG_ENOTFOUND = -5,    /* unknown ticket / missing PCSR                 */
-  FSIG_EAGAIN = -6        /* server: not finished yet / queue full         */
+  FSIG_EAGAIN = -6,       /* server: not finished yet / queue full         */
+  FSIG_ELIMIT = -7        /* the context's work limit was exceeded         */
 };
 
 enum { FSIG_ALG_MMATCH = 1, FSIG_ALG_GSI = 2 };
@@ -88,11 +89,13 @@ void fsig_graph_free(fsig_ctx* ctx, fsig_graph* g);
  *        FSIG_ALG_GSI: signature filter, join over the resident PCSR.
  *        Replaces GSI::match (match/GSI.cu:329-358).
  *   mode FSIG_MODE_COUNT: *count_out = number of embeddings (u64).
- *        FSIG_MODE_ENUMERATE: embeddings are materialised in HBM; at most
- *        max_embeddings of them (0 = no bound), *count_out = number
- *        enumerated; the first min(count, emb_cap) rows are copied to
- *        emb_out (host, k words per row, query-vertex order) when
- *        emb_out != NULL.  Which rows form a bounded prefix is unspecified.
+ *        FSIG_MODE_ENUMERATE: embeddings are materialised in HBM; the first
+ *        min(count, emb_cap) rows are copied to emb_out (host, k words per
+ *        row, query-vertex order) when emb_out != NULL.
+ *   max_embeddings (0 = no bound) bounds both modes: the work stops once
+ *        that many embeddings are found and *count_out = min(count,
+ *        max_embeddings); pass L+1 to ask "more than L?".  Which embeddings
+ *        form a bounded prefix is unspecified.
  */
 int fsig_match(fsig_ctx* ctx, const fsig_graph* g, uint32_t k, const int32_t* qlabels,
                const uint32_t* qoff, const uint32_t* qadj, const uint32_t* qsig, int alg, int mode,
@@ -110,8 +113,12 @@ int fsig_match(fsig_ctx* ctx, const fsig_graph* g, uint32_t k, const int32_t* ql
 int fsig_match_batch(fsig_ctx* ctx, const fsig_graph* g, uint32_t nq, const uint32_t* k,
                      const int32_t* const* qlabels, const uint32_t* const* qoff,
                      const uint32_t* const* qadj, uint64_t* counts_out, double* join_ms);
-/* Configuration the autotuner chose for the last batch (lds_tile 0: none). */
-int fsig_last_batch_tune(const fsig_ctx* ctx, uint32_t* lanes_per_row, uint32_t* lds_tile);
+/* Configuration the autotuner chose for the last batch: engine 1 = lockstep
+ * batched join (every query's level p in one count/scan/write sequence),
+ * 2 = persistent work-queue walk; lanes per row or group; LDS tile (0:
+ * none).  With fsig_set_tune(lanes_per_row != 0) the batch uses strategy 0:
+ * lockstep, 1: persistent, 2: persistent with LDS staging. */
+int fsig_last_batch_tune(const fsig_ctx* ctx, uint32_t* lanes_per_row, uint32_t* lds_tile, uint32_t* engine);
 
 /* Per-kernel-family statistics of the context, accumulated since the last
  * fsig_stats_reset.  With profiling on, every launch of the family is
@@ -152,6 +159,12 @@ typedef struct fsig_tune {
 } fsig_tune;
 int fsig_set_tune(fsig_ctx* ctx, const fsig_tune* t);
 int fsig_get_tune(const fsig_ctx* ctx, fsig_tune* t);
+
+/* Work limit of fsig_match: when more than max_rows partial embeddings
+ * (rows of all intermediate tables together) would be produced, the match
+ * stops and returns FSIG_ELIMIT.  0 removes the limit.  Lets a caller reject
+ * queries whose search space is too large before paying for it. */
+int fsig_set_work_limit(fsig_ctx* ctx, uint64_t max_rows);
 
 /* Achievable-bandwidth probe: a plain copy kernel (16-byte loads and stores)
  * of `bytes` from one arena buffer to another, run `iters` times on the

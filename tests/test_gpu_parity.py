@@ -74,18 +74,23 @@ def test_random_walk_queries_labeled(gpu_ctx, seed):
 
 
 def test_unlabeled_patterns_count(gpu_ctx):
-    g = F.HostGraph.rmat(12, 40000, seed=7)
-    off, adj, _ = g.arrays()
-    lab = np.zeros(g.n, np.int32)
-    dg = gpu_ctx.upload(g, pcsr=True)
+    # the house runs on the smaller graph: its count on the larger one is
+    # 1.5e10 and takes the oracle minutes
     pats = {"triangle": (3, [(0, 1), (1, 2), (0, 2)]), "cycle4": (4, [(0, 1), (1, 2), (2, 3), (3, 0)]),
             "clique4": (4, [(a, b) for a in range(4) for b in range(a + 1, 4)]),
             "house": (5, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0), (1, 4)])}
-    for name, (k, e) in pats.items():
-        q = F.Query(k, np.zeros(k, np.int32), e)
-        want, _ = O.match(q.labels, q.adjm(), off, adj, lab, threads=8)
-        for alg in (F.ALG_MMATCH, F.ALG_GSI):
-            assert F.match(dg, q, alg, F.MODE_COUNT).count == want, (name, alg)
+    for scale, m, names in ((12, 40000, ("triangle", "cycle4", "clique4")), (10, 7000, ("house",))):
+        g = F.HostGraph.rmat(scale, m, seed=7)
+        off, adj, _ = g.arrays()
+        lab = np.zeros(g.n, np.int32)
+        dg = gpu_ctx.upload(g, pcsr=True)
+        for name in names:
+            k, e = pats[name]
+            q = F.Query(k, np.zeros(k, np.int32), e)
+            want, _ = O.match(q.labels, q.adjm(), off, adj, lab, threads=8)
+            for alg in (F.ALG_MMATCH, F.ALG_GSI):
+                assert F.match(dg, q, alg, F.MODE_COUNT).count == want, (name, alg)
+        dg.free()
 
 
 def test_candidate_sets_equal_oracle_refinement(gpu_ctx):
@@ -145,6 +150,18 @@ def test_edge_cases(gpu_ctx):
         rc = L.fsig_match(gpu_ctx._h, dg._h, k, F._ptr(labels, F._i32p), F._ptr(qo), F._ptr(qa),
                           F._ptr(None), F.ALG_MMATCH, F.MODE_COUNT, 0, F.C.byref(cnt), F._ptr(None), 0)
         assert rc in (F.FSIG_EINVAL, F.FSIG_EUNSUPPORTED), (qoff, qadj, rc)
+
+
+def test_bounded_count(gpu_ctx):
+    g = F.HostGraph.chung_lu(5000, 30000, 2.3, 63).set_labels("uniform", 2, seed=64)
+    dg = gpu_ctx.upload(g)
+    q = F.Query(4, [0, 1, 0, 1], [(0, 1), (1, 2), (2, 3)])
+    total = F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT).count
+    assert total > 1000
+    for bound in (1, 999, total - 1, total, total + 1):
+        for alg in (F.ALG_MMATCH, F.ALG_GSI):
+            pass
+        assert F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT, max_embeddings=bound).count == min(total, bound)
 
 
 def test_bounded_enumeration(gpu_ctx):
