@@ -105,13 +105,21 @@ HIP_SYMBOLS = [
     "fsig_graph_upload_pcsr", "fsig_graph_free", "fsig_match", "fsig_set_profiling",
     "fsig_stats_reset", "fsig_kernel_stats", "fsig_last_candidates", "fsig_last_peak_bytes",
     "fsig_set_tune", "fsig_get_tune", "fsig_stream_copy", "fsig_match_batch", "fsig_last_batch_tune",
-    "fsig_set_work_limit",
+    "fsig_set_work_limit", "fsig_server_create", "fsig_server_submit", "fsig_server_wait",
+    "fsig_server_get_stats", "fsig_server_projected", "fsig_server_last_error", "fsig_server_destroy",
+    "fsig_project_bytes", "fsig_graph_label_pairs",
 ]
 
 
 class Tune(C.Structure):
     _fields_ = [("lanes_per_row", C.c_uint32), ("strategy", C.c_uint32), ("lds_tile", C.c_uint32),
                 ("refine_passes", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("batches", C.c_uint64),
+                ("fallbacks", C.c_uint64), ("max_batch_seen", C.c_uint64), ("max_concurrent", C.c_uint32),
+                ("pad", C.c_uint32), ("busy_ms", C.c_double), ("wall_ms", C.c_double)]
 
 
 def hip_lib():
@@ -153,6 +161,19 @@ def _optional_bindings(L):
     L.fsig_match_batch.argtypes = [_vp, _vp, C.c_uint32, _u32p, C.POINTER(_i32p), C.POINTER(_u32p),
                                    C.POINTER(_u32p), _u64p, C.POINTER(C.c_double)]
     L.fsig_last_batch_tune.argtypes = [_vp, _u32p, _u32p, _u32p]
+    L.fsig_server_create.argtypes = [C.c_int, _vp, C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(Tune),
+                                     C.POINTER(_vp)]
+    L.fsig_server_submit.argtypes = [_vp, C.c_uint32, _i32p, _u32p, _u32p, _u64p]
+    L.fsig_server_wait.argtypes = [_vp, C.c_uint64, C.c_int64, _u64p, C.POINTER(C.c_double)]
+    L.fsig_server_get_stats.argtypes = [_vp, C.POINTER(ServerStats)]
+    L.fsig_server_projected.argtypes = [_vp, C.c_uint32, _i32p, _u32p, _u32p, _u64p]
+    L.fsig_server_last_error.restype = C.c_char_p
+    L.fsig_server_last_error.argtypes = [_vp]
+    L.fsig_server_destroy.argtypes = [_vp]
+    L.fsig_server_destroy.restype = None
+    L.fsig_project_bytes.argtypes = [C.c_uint32, C.c_uint32, _u64p, _u64p, C.c_uint32, _i32p, _u32p, _u32p,
+                                     _u64p]
+    L.fsig_graph_label_pairs.argtypes = [_vp, _vp, _u64p]
 
 
 # ---------------------------------------------------------------------------
@@ -327,6 +348,7 @@ class Context:
         if rc:
             raise FsigError(rc, f"cannot create a context on device {device}")
         self._h = h
+        self.device = device
 
     def close(self):
         if getattr(self, "_h", None):
@@ -474,6 +496,104 @@ def match_batch(dg: "DeviceGraph", queries: Sequence[Query]) -> Tuple[np.ndarray
 
 
 BATCH_ENGINES = {1: "lockstep", 2: "persistent"}
+
+
+def _query_arrays(q: "Query"):
+    off, adj = q.csr()
+    lab = np.ascontiguousarray(q.labels, dtype=np.int32)
+    adj = adj if len(adj) else np.zeros(1, np.uint32)
+    return lab, off, adj
+
+
+def project_bytes(n: int, label_count: np.ndarray, pair_edges: np.ndarray, q: "Query") -> int:
+    """fsig_project_bytes: the arena bytes the served loop admits `q` with on a
+    graph of n vertices with these label and label-pair adjacency counts."""
+    L = hip_lib()
+    lc = np.ascontiguousarray(label_count, dtype=np.uint64)
+    pe = np.ascontiguousarray(pair_edges, dtype=np.uint64).reshape(-1)
+    if pe.size != lc.size * lc.size:
+        raise FsigError(FSIG_EINVAL, "pair_edges must hold num_labels^2 counts")
+    lab, off, adj = _query_arrays(q)
+    out = C.c_uint64()
+    rc = L.fsig_project_bytes(n, lc.size, _ptr(lc, _u64p), _ptr(pe, _u64p), q.k, _ptr(lab, _i32p), _ptr(off),
+                              _ptr(adj), C.byref(out))
+    if rc:
+        raise FsigError(rc, "fsig_project_bytes failed")
+    return out.value
+
+
+def label_pairs(dg: "DeviceGraph") -> np.ndarray:
+    """[L][L] adjacency-entry counts between labels of a resident graph."""
+    ctx = dg.ctx
+    L = dg.host.num_labels
+    out = np.zeros(L * L, np.uint64)
+    ctx.check(ctx._L.fsig_graph_label_pairs(ctx._h, dg._h, _ptr(out, _u64p)))
+    return out.reshape(L, L)
+
+
+class Server:
+    """fsig_server: continuous batching of count queries over a resident
+    graph (workers with their own stream and arena slice; admission by
+    projected intermediate-table bytes)."""
+
+    def __init__(self, dg: "DeviceGraph", workers: int = 4, hbm_budget: int = 0, max_batch: int = 64,
+                 tune: Tune = None):
+        self._L = hip_lib()
+        self.dg = dg  # keeps the graph alive
+        h = _vp()
+        rc = self._L.fsig_server_create(dg.ctx.device, dg._h, workers, hbm_budget, max_batch,
+                                        C.byref(tune) if tune is not None else None, C.byref(h))
+        if rc:
+            raise FsigError(rc, "fsig_server_create failed")
+        self._h = h
+
+    def submit(self, q: "Query") -> int:
+        lab, off, adj = _query_arrays(q)
+        t = C.c_uint64()
+        rc = self._L.fsig_server_submit(self._h, q.k, _ptr(lab, _i32p), _ptr(off), _ptr(adj), C.byref(t))
+        if rc:
+            raise FsigError(rc, self._L.fsig_server_last_error(self._h).decode())
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1) -> Tuple[int, float]:
+        """(count, latency ms) of a ticket; FsigError(FSIG_EAGAIN) on timeout."""
+        cnt, lat = C.c_uint64(), C.c_double()
+        rc = self._L.fsig_server_wait(self._h, ticket, timeout_ms, C.byref(cnt), C.byref(lat))
+        if rc:
+            raise FsigError(rc, self._L.fsig_server_last_error(self._h).decode() or "server wait failed")
+        return cnt.value, lat.value
+
+    def projected(self, q: "Query") -> int:
+        lab, off, adj = _query_arrays(q)
+        out = C.c_uint64()
+        rc = self._L.fsig_server_projected(self._h, q.k, _ptr(lab, _i32p), _ptr(off), _ptr(adj), C.byref(out))
+        if rc:
+            raise FsigError(rc, "fsig_server_projected failed")
+        return out.value
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        rc = self._L.fsig_server_get_stats(self._h, C.byref(st))
+        if rc:
+            raise FsigError(rc, "fsig_server_get_stats failed")
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_ if f != "pad"}
+
+    def close(self):
+        if self._h:
+            self._L.fsig_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def last_batch_tune(ctx: "Context") -> Tuple[int, int, str]:

@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 
 #include "common.h"
 #include "device.h"
@@ -1019,7 +1020,10 @@ struct StageClock {
 };
 
 // Per-context cache of tuning decisions, keyed by a coarse batch shape.
+// Contexts may run batches from several host threads at once (the served
+// loop does), so the cache is locked.
 static std::map<std::pair<const fsig_ctx*, uint64_t>, TuneChoice> g_tuned;
+static std::mutex g_tuned_mu;
 
 extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint32_t* ks,
                                 const int32_t* const* qlabels, const uint32_t* const* qoff,
@@ -1460,6 +1464,7 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
       k_iota_multi<<<bgrid(c, R, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(dqb, nl, rows, R);
       std::vector<MLevel> ml(nl);
       for (uint32_t p = 1; p < maxk && R > 0; p++) {
+        if (clk.on) clk.mark("lockstep level start");
         bool any_more = false;
         for (uint32_t j = 0; j < nl; j++) {
           ml[j] = make_level(live[j], p);
@@ -1480,6 +1485,11 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
         if (rc2) { A.release(mk); return rc2; }
         uint64_t T;
         if ((rc2 = read_u64(c, off + R, &T, 1))) { A.release(mk); return rc2; }
+        if (clk.on) {
+          fprintf(stderr, "[fsig batch]   level %u: %llu rows -> %llu\n", p, (unsigned long long)R,
+                  (unsigned long long)T);
+          clk.mark("  count + scan");
+        }
         if (T == 0) break;
         out = (uint32_t*)A.alloc(4ull * (p + 1) * T);
         if (!out) { A.release(mk); return set_err(c, FSIG_ENOMEM, "lockstep: a level does not fit the arena"); }
@@ -1561,10 +1571,13 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
     best = TuneChoice{c->tune.lanes_per_row, c->tune.strategy == 2, c->tune.lds_tile ? c->tune.lds_tile : 256,
                       c->tune.strategy != 1 && c->tune.strategy != 2};
   } else {
-    auto it = g_tuned.find({c, shape});
-    if (it != g_tuned.end()) {
-      best = it->second;
-    } else {
+    bool cached = false;
+    {
+      std::lock_guard<std::mutex> lk(g_tuned_mu);
+      auto it = g_tuned.find({c, shape});
+      if (it != g_tuned.end()) { best = it->second; cached = true; }
+    }
+    if (!cached) {
       // sample: every 8th live query; both engines, several shapes each
       std::vector<uint32_t> sample;
       for (size_t j = 0; j < all_q.size(); j += 8) sample.push_back(all_q[j]);
@@ -1580,6 +1593,7 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
         if (rc) return rc;
         if (ms < best_ms) { best_ms = ms; best = tc; }
       }
+      std::lock_guard<std::mutex> lk(g_tuned_mu);
       g_tuned[{c, shape}] = best;
     }
   }

@@ -71,6 +71,17 @@ def c4_graph() -> HostGraph:
     return HostGraph.chung_lu(1_000_000, 10_000_000, 2.3, 5).set_labels("zipf", 32, 1.2, 6)
 
 
+# Walks whose query has more than 1e8 embeddings, or whose plain join would
+# hold more than 2e7 partial embeddings, are redrawn (the next seed).
+C4_QUERIES = dict(count=256, kmin=4, kmax=16, seed=4, max_count=10 ** 8, max_rows=2 * 10 ** 7)
+
+
+def c4_queries(g: HostGraph, dg) -> List[Query]:
+    p = C4_QUERIES
+    return [q for _, q in walk_queries(g, p["count"], p["kmin"], p["kmax"], p["seed"], dg=dg,
+                                       max_count=p["max_count"], max_rows=p["max_rows"])]
+
+
 def walk_queries(g: HostGraph, count: int, kmin: int, kmax: int, seed: int, log_uniform: bool = False,
                  max_edges: int = 10 ** 6, dg=None, max_count: int = 0,
                  max_rows: int = 0) -> List[Tuple[str, Query]]:
@@ -115,3 +126,22 @@ def walk_queries(g: HostGraph, count: int, kmin: int, kmax: int, seed: int, log_
 
 def c5_graph(scale: int = 24, m: int = 200_000_000, seed: int = 8) -> HostGraph:
     return HostGraph.rmat(scale, m, 0.57, 0.19, 0.19, seed).set_labels("uniform", 64, seed=9)
+
+
+# C5 request stream: k = round(exp(U(ln 4, ln 32))), alternating dense and
+# sparse walks (redrawn like C4's above the same bounds), Poisson arrivals at
+# `qps` (exponential gaps from their own seed).
+C5_STREAM = dict(count=512, kmin=4, kmax=32, seed=11, arrival_seed=12, qps=2000.0, max_count=10 ** 8,
+                 max_rows=2 * 10 ** 7)
+
+
+def c5_stream(g: HostGraph, dg, count: int = 0, qps: float = 0.0) -> List[Tuple[float, Query]]:
+    """(arrival time in s from the stream start, query) pairs."""
+    p = C5_STREAM
+    count = count or p["count"]
+    qps = qps or p["qps"]
+    qs = [q for _, q in walk_queries(g, count, p["kmin"], p["kmax"], p["seed"], log_uniform=True, dg=dg,
+                                     max_count=p["max_count"], max_rows=p["max_rows"])]
+    gaps = np.random.default_rng(p["arrival_seed"]).exponential(1.0 / qps, size=len(qs))
+    t = np.cumsum(gaps) - gaps[0]
+    return list(zip(t.tolist(), qs))

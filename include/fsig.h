@@ -173,6 +173,61 @@ int fsig_set_work_limit(fsig_ctx* ctx, uint64_t max_rows);
  * reference. */
 int fsig_stream_copy(fsig_ctx* ctx, uint64_t bytes, uint32_t iters, double* gbps);
 
+/* ---- Served request loop (continuous batching of count queries).
+ *
+ * `workers` worker contexts on `device` (each its own HIP stream and an arena
+ * of hbm_budget_bytes / workers; 0 = 3/4 of the free device memory) share
+ * the resident graph g.  A worker that is free takes the requests waiting at
+ * that moment, oldest first, up to max_batch of them and while their
+ * projected intermediate-table bytes (fsig_project_bytes) fit 3/4 of its
+ * arena (the first request is always taken), and counts them as one batch
+ * with fsig_match_batch's kernels.  Requests that arrive while batches run
+ * wait only for the next free worker, and the batches of different workers
+ * overlap on the device.  A batch that still runs out of arena falls back
+ * to fsig_match per query, whose join is memory-blocked.  tune (NULL:
+ * lockstep join, 16 lanes per row, no autotuning) is set on every worker.
+ * Replaces the reference's one query per process with a static device
+ * budget (main/run.cc:101-140, memory_manager/memory_manager.h:220-242). */
+typedef struct fsig_server_stats {
+  uint64_t submitted, completed, batches, fallbacks, max_batch_seen;
+  uint32_t max_concurrent; /* most batches in flight at once */
+  uint32_t pad;
+  double busy_ms;          /* summed batch wall time over the workers */
+  double wall_ms;          /* first submit to last completion */
+} fsig_server_stats;
+int fsig_server_create(int device, const fsig_graph* g, uint32_t workers, uint64_t hbm_budget_bytes,
+                       uint32_t max_batch, const fsig_tune* tune, fsig_server** out);
+/* Queues one count query (alg 1 semantics; the arrays are copied) and
+ * returns its ticket.  The submit time starts the query's latency. */
+int fsig_server_submit(fsig_server* s, uint32_t k, const int32_t* qlabels, const uint32_t* qoff,
+                       const uint32_t* qadj, uint64_t* ticket);
+/* Waits up to timeout_ms (< 0: without limit) for a ticket; returns the
+ * query's status (FSIG_EAGAIN: not finished, the ticket stays valid) and on
+ * completion its count and submit-to-completion latency.  A completed
+ * ticket is released by the call that returns it. */
+int fsig_server_wait(fsig_server* s, uint64_t ticket, int64_t timeout_ms, uint64_t* count, double* latency_ms);
+int fsig_server_get_stats(fsig_server* s, fsig_server_stats* out);
+/* Projected bytes the server admits the query with. */
+int fsig_server_projected(fsig_server* s, uint32_t k, const int32_t* qlabels, const uint32_t* qoff,
+                          const uint32_t* qadj, uint64_t* bytes);
+const char* fsig_server_last_error(fsig_server* s);
+/* Finishes every queued request, stops the workers and frees them. */
+void fsig_server_destroy(fsig_server* s);
+
+/* Admission projection (host only): arena bytes of the batched engine for
+ * one count query on a graph of n vertices whose label l has label_count[l]
+ * vertices and whose adjacency holds pair_edges[a * num_labels + b] entries
+ * from a label-a to a label-b vertex.  Setup tables plus every stored join
+ * level; level sizes follow an independence estimate (first backward
+ * neighbour: mean label-matching degree; each further one: edge
+ * probability).  Generalises the reference's per-level block sizing
+ * (memory_manager/memory_manager.h:220-242) to a whole query. */
+int fsig_project_bytes(uint32_t n, uint32_t num_labels, const uint64_t* label_count, const uint64_t* pair_edges,
+                       uint32_t k, const int32_t* qlabels, const uint32_t* qoff, const uint32_t* qadj,
+                       uint64_t* bytes);
+/* Label-pair adjacency counts of a resident graph (num_labels^2 u64). */
+int fsig_graph_label_pairs(fsig_ctx* ctx, const fsig_graph* g, uint64_t* pair_edges);
+
 #ifdef __cplusplus
 }
 #endif
