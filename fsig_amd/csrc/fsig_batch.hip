@@ -53,9 +53,13 @@ __device__ __forceinline__ uint32_t find_flat(const T* __restrict__ d, uint32_t 
   return lo;
 }
 
+// With nlf non-null, global class gc also needs, for each entry e of
+// req[gc * MAXQ .. + nreq[gc]), nlf[v * L + (e >> 8)] >= (e & 255).
 __global__ void __launch_bounds__(FSIG_BLOCK) k_bldf(const BSeg* __restrict__ S, uint32_t ns, uint64_t total,
                                                      const uint32_t* __restrict__ cdeg,
-                                                     const uint32_t* __restrict__ off) {
+                                                     const uint32_t* __restrict__ off, const uint8_t* __restrict__ nlf,
+                                                     uint32_t L, const uint32_t* __restrict__ req,
+                                                     const uint32_t* __restrict__ nreq) {
   for (uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += (uint64_t)gridDim.x * blockDim.x) {
     const BSeg& s = S[find_flat(S, ns, f)];
     const uint32_t v = s.src[f - s.flat];
@@ -64,7 +68,15 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bldf(const BSeg* __restrict__ S,
     while (cm) {
       const uint32_t c = __builtin_ctz(cm);
       cm &= cm - 1;
-      if (d >= cdeg[s.cls_base + c]) m |= 1u << c;
+      const uint32_t gc = s.cls_base + c;
+      if (d < cdeg[gc]) continue;
+      bool ok = true;
+      if (nlf) {
+        const uint8_t* row = nlf + (uint64_t)v * L;
+        const uint32_t* rq = req + (uint64_t)gc * MAXQ;
+        for (uint32_t i = 0; i < nreq[gc] && ok; i++) ok = row[rq[i] >> 8] >= (rq[i] & 255u);
+      }
+      if (ok) m |= 1u << c;
     }
     s.mask[v] = m;
   }
@@ -766,6 +778,15 @@ struct MLevel {
   CandEdgeSrc S;
   uint32_t last;  // level p+1 is the query's last: count only
   uint32_t qid;   // batch index of the query
+  uint32_t tail;  // level p+2 is the last: both counted in one pass (L2, S2)
+  JoinLevel L2;
+  CandEdgeSrc S2;
+};
+
+template <int G>
+struct PlainShared {
+  static constexpr int GPB = FSIG_BLOCK / G;
+  uint32_t s1[GPB][MAXQ], n1[GPB][MAXQ], cf1[GPB][MAXQ];
 };
 
 template <int G, bool WRITE>
@@ -779,14 +800,18 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join_multi(const MLevel* __restr
                                                            unsigned long long* __restrict__ stats) {
   constexpr int GPB = FSIG_BLOCK / G;
   constexpr int NR = (MAXQ + G - 1) / G;
-  __shared__ uint32_t sh_s[GPB][MAXQ];
-  __shared__ uint32_t sh_n[GPB][MAXQ];
-  __shared__ uint32_t sh_cf[GPB][MAXQ];
+  // the count pass of G >= 8 also runs fused tails (their LDS tiles)
+  constexpr bool TAIL = !WRITE && G >= 8;
+  using Sh = typename std::conditional<TAIL, TailShared<G>, PlainShared<G>>::type;
+  __shared__ Sh shm;
+  auto& sh_s = shm.s1;
+  auto& sh_n = shm.n1;
+  auto& sh_cf = shm.cf1;
   const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
   const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
   const uint64_t below = (1ull << lane) - 1;
   const uint64_t ngroups = (uint64_t)gridDim.x * GPB;
-  uint64_t my_bytes = 0, my_units = 0;
+  unsigned long long my_bytes = 0, my_units = 0;
   uint32_t cur_q = 0xffffffffu;
   unsigned long long acc = 0;
   for (uint64_t r = (uint64_t)xcd_block() * GPB + gw; r < nrows; r += ngroups) {
@@ -798,13 +823,32 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join_multi(const MLevel* __restr
     const MLevel& M = ML[lo];
     const JoinLevel& L = M.L;
     const bool last = M.last != 0;
-    if (WRITE && last) continue;  // counted in the count pass
+    if (WRITE && (last || off[r + 1] == off[r])) continue;  // counted in the count pass, or no children
     uint32_t rv[NR];
     const uint32_t* row = rows + r * p;
 #pragma unroll
     for (int j = 0; j < NR; j++) {
       const uint32_t col = j * G + gl;
       rv[j] = col < p ? row[col] : 0u;
+    }
+    if constexpr (TAIL) {
+      if (M.tail) {
+        uint64_t kids = 0;
+        bool defer = false;
+        const uint64_t leaves = group_sum_u64<G>(tail_row<G>(L, M.S, M.L2, M.S2, rv, shm.at(gw), gl, gmask, below,
+                                                             kids, defer, my_bytes));
+        if (gl == 0) {
+          my_units += 1;
+          if (M.qid != cur_q) {
+            if (cur_q != 0xffffffffu && acc) atomicAdd(&qcounts[cur_q], acc);
+            cur_q = M.qid;
+            acc = 0;
+          }
+          acc += leaves;
+          cnt[r] = defer ? (uint32_t)kids : 0u;  // a deferred row's children are written
+        }
+        continue;
+      }
     }
     uint32_t dlen = L.wsize, d = 0;
     uint64_t elems = 0;
@@ -844,7 +888,7 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join_multi(const MLevel* __restr
         hit += __popcll(__ballot(in) & gmask);
       }
       base = dlen - hit;
-      my_bytes += 4ull * ncf * (32 - __builtin_clz(dlen | 1));
+      if (gl == 0) my_bytes += 4ull * ncf * (32 - __builtin_clz(dlen | 1));
       elems = 0;
     }
     uint64_t child0 = WRITE ? off[r] : 0;
@@ -891,9 +935,11 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join_multi(const MLevel* __restr
     }
   }
   if (!WRITE && gl == 0 && cur_q != 0xffffffffu && acc) atomicAdd(&qcounts[cur_q], acc);
-  if (stats && gl == 0) {
-    atomicAdd(&stats[0], (unsigned long long)my_bytes);
-    atomicAdd(&stats[1], (unsigned long long)my_units);
+  my_bytes = wave_sum_u64(my_bytes);
+  my_units = wave_sum_u64(my_units);
+  if (stats && lane == 0) {
+    atomicAdd(&stats[0], my_bytes);
+    atomicAdd(&stats[1], my_units);
   }
 }
 
@@ -1106,9 +1152,25 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
   const uint32_t* dcdeg = upload_table(c, cdeg);
   const BSeg* dsegs = upload_table(c, segs_nz);
   if (!dcdeg || !dsegs) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted for tables");
+  const bool use_nlf = g->d_nlf && c->tune.nlf_filter == 0;
+  const uint32_t* dreq = nullptr;
+  const uint32_t* dnreq = nullptr;
+  if (use_nlf) {
+    std::vector<uint32_t> req((size_t)std::max<uint32_t>(ncls, 1) * MAXQ, 0), nreq(std::max<uint32_t>(ncls, 1), 0);
+    for (uint32_t i = 0; i < nq; i++)
+      for (uint32_t a = 0; a < plans[i].nclass; a++) {
+        const uint32_t gc = cls_base[i] + a;
+        nreq[gc] = plans[i].nlf_n[a];
+        for (uint32_t j = 0; j < plans[i].nlf_n[a]; j++) req[(size_t)gc * MAXQ + j] = plans[i].nlf_req[a][j];
+      }
+    dreq = upload_table(c, req);
+    dnreq = upload_table(c, nreq);
+    if (!dreq || !dnreq) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted for tables");
+  }
   if (flat)
     k_bldf<<<bgrid(c, flat, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(dsegs, (uint32_t)segs_nz.size(), flat, dcdeg,
-                                                                     g->d_off);
+                                                                     g->d_off, use_nlf ? g->d_nlf : nullptr,
+                                                                     g->num_labels, dreq, dnreq);
   std::vector<uint32_t*> cand_a(ncls, nullptr), cand_b(ncls, nullptr);
   std::vector<uint32_t> size(ncls, 0);
   for (uint32_t i = 0; i < nq; i++) {
@@ -1413,32 +1475,42 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
   // ---- join engines
   // Lockstep batched BFS over a set of live queries; counts land in dcounts.
   const size_t join_mark = A.mark();
-  auto make_level = [&](uint32_t i, uint32_t p) {
+  auto fill_level = [&](uint32_t i, uint32_t p, JoinLevel& L, CandEdgeSrc& S) {
+    const QueryPlan& P = plans[i];
+    const uint32_t w = P.order[p];
+    L.p = p;
+    L.wcand = cand[cls_base[i] + P.cls[w]];
+    L.wsize = size[cls_base[i] + P.cls[w]];
+    for (uint32_t cp = 0; cp < p; cp++) {
+      const uint32_t u = P.order[cp];
+      if ((P.adjm[w] >> u) & 1u) {
+        const int pi = pair_of[i][P.cls[u] * MAXQ + P.cls[w]];
+        L.bcol[L.nb] = cp;
+        S.off[L.nb] = pairs[pi].off_out;
+        S.lst[L.nb] = lst_all + pbase_h[pi];
+        L.nb++;
+      } else if (P.label[u] == P.label[w]) {
+        const uint32_t j = L.ncf++;
+        L.cfcol[j] = cp;
+        if (P.cls[u] == P.cls[w]) L.cfsame |= 1u << j;
+        else { L.cfcand[j] = cand[cls_base[i] + P.cls[u]]; L.any_cross = 1; }
+      }
+    }
+  };
+  // Level p of query i; with `tail` the query's last two levels (p + 2 == k)
+  // are counted in one pass when the last vertex has a backward neighbour.
+  auto make_level = [&](uint32_t i, uint32_t p, bool tail) {
     MLevel M;
     std::memset(&M, 0, sizeof(M));
     M.qid = i;
     const QueryPlan& P = plans[i];
     if (p >= P.k) { M.last = 1; return M; }
-    const uint32_t w = P.order[p];
-    M.L.p = p;
-    M.L.wcand = cand[cls_base[i] + P.cls[w]];
-    M.L.wsize = size[cls_base[i] + P.cls[w]];
-    for (uint32_t cp = 0; cp < p; cp++) {
-      const uint32_t u = P.order[cp];
-      if ((P.adjm[w] >> u) & 1u) {
-        const int pi = pair_of[i][P.cls[u] * MAXQ + P.cls[w]];
-        M.L.bcol[M.L.nb] = cp;
-        M.S.off[M.L.nb] = pairs[pi].off_out;
-        M.S.lst[M.L.nb] = lst_all + pbase_h[pi];
-        M.L.nb++;
-      } else if (P.label[u] == P.label[w]) {
-        const uint32_t j = M.L.ncf++;
-        M.L.cfcol[j] = cp;
-        if (P.cls[u] == P.cls[w]) M.L.cfsame |= 1u << j;
-        else { M.L.cfcand[j] = cand[cls_base[i] + P.cls[u]]; M.L.any_cross = 1; }
-      }
-    }
+    fill_level(i, p, M.L, M.S);
     M.last = p + 1 == P.k;
+    if (tail && p + 2 == P.k) {
+      fill_level(i, p + 1, M.L2, M.S2);
+      M.tail = tail_applies(M.L2) ? 1 : 0;
+    }
     return M;
   };
   unsigned long long* stats_ls = family_counters(c, FSIG_K_BATCH);
@@ -1463,12 +1535,17 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
       uint64_t R = qb[nl];
       k_iota_multi<<<bgrid(c, R, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(dqb, nl, rows, R);
       std::vector<MLevel> ml(nl);
+      uint64_t prev_bytes = 0;  // debug timing: device byte counter before the level
+      if (clk.on && stats_ls) (void)read_u64(c, stats_ls, &prev_bytes, 1);
       for (uint32_t p = 1; p < maxk && R > 0; p++) {
-        if (clk.on) clk.mark("lockstep level start");
+        if (clk.on) {
+          clk.mark("lockstep level start");
+          if (stats_ls) (void)read_u64(c, stats_ls, &prev_bytes, 1);
+        }
         bool any_more = false;
         for (uint32_t j = 0; j < nl; j++) {
-          ml[j] = make_level(live[j], p);
-          any_more |= ml[j].last == 0;
+          ml[j] = make_level(live[j], p, G >= 8 && c->tune.fused_tail == 0);
+          any_more |= ml[j].last == 0;  // tail rows may defer children to the last level
         }
         FSIG_HIP(c, hipMemcpyAsync(dml, ml.data(), sizeof(MLevel) * nl, hipMemcpyHostToDevice, c->stream));
         uint32_t* cnt = (uint32_t*)A.alloc(4ull * R);
@@ -1486,8 +1563,11 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
         uint64_t T;
         if ((rc2 = read_u64(c, off + R, &T, 1))) { A.release(mk); return rc2; }
         if (clk.on) {
-          fprintf(stderr, "[fsig batch]   level %u: %llu rows -> %llu\n", p, (unsigned long long)R,
-                  (unsigned long long)T);
+          uint64_t st2[2] = {0, 0};
+          if (stats_ls && read_u64(c, stats_ls, st2, 2) == FSIG_OK) {
+            fprintf(stderr, "[fsig batch]   level %u: %llu rows -> %llu, count-pass bytes/row %.1f\n", p,
+                    (unsigned long long)R, (unsigned long long)T, R ? (double)(st2[0] - prev_bytes) / R : 0.0);
+          }
           clk.mark("  count + scan");
         }
         if (T == 0) break;

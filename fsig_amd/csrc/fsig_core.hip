@@ -122,6 +122,32 @@ int read_u64(fsig_ctx* c, const void* dev, uint64_t* host, size_t count) {
 
 // ---------------------------------------------------------------------------
 // C ABI
+// Neighbour-label profile: one wavefront per vertex counts its neighbours'
+// labels in LDS and stores them saturated to a byte.
+__global__ void __launch_bounds__(FSIG_BLOCK) k_nlf_build(const uint32_t* __restrict__ off,
+                                                          const uint32_t* __restrict__ adj,
+                                                          const int32_t* __restrict__ lab, uint32_t n, uint32_t L,
+                                                          uint8_t* __restrict__ nlf) {
+  extern __shared__ uint32_t hist[];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t* h = hist + wave * L;
+  for (uint64_t v = (uint64_t)blockIdx.x * (FSIG_BLOCK / 64) + wave; v < n;
+       v += (uint64_t)gridDim.x * (FSIG_BLOCK / 64)) {
+    for (uint32_t l = lane; l < L; l += 64) h[l] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t j = off[v] + lane; j < off[v + 1]; j += 64) atomicAdd(&h[lab[adj[j]]], 1u);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t l = lane; l < L; l += 64) nlf[v * L + l] = (uint8_t)(h[l] > 255u ? 255u : h[l]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
 extern "C" {
 
 int fsig_ctx_create(int device, uint64_t budget, fsig_ctx** out) {
@@ -237,6 +263,23 @@ int fsig_graph_upload_csr(fsig_ctx* c, uint32_t n, const uint32_t* off, const ui
     fsig_graph_free(c, g);
     return set_err(c, FSIG_ENOMEM, "upload_csr: device allocation or copy failed");
   }
+  // neighbour-label profile (a filter of alg 1), when labels are few enough
+  // for one byte row per vertex
+  if (num_labels >= 2 && num_labels <= 255 && (uint64_t)n * num_labels <= (16ull << 30)) {
+    if (hipMalloc((void**)&g->d_nlf, std::max<uint64_t>((uint64_t)n * num_labels, 4)) != hipSuccess) {
+      fsig_graph_free(c, g);
+      return set_err(c, FSIG_ENOMEM, "upload_csr: no memory for the neighbour-label profile");
+    }
+    if (n) {
+      k_nlf_build<<<(uint32_t)std::min<uint64_t>((n + 3) / 4, (uint64_t)c->num_cus * 64), FSIG_BLOCK,
+                    (FSIG_BLOCK / 64) * num_labels * sizeof(uint32_t), c->stream>>>(g->d_off, g->d_adj, g->d_labels,
+                                                                                   n, num_labels, g->d_nlf);
+      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+        fsig_graph_free(c, g);
+        return set_err(c, FSIG_EHIP, "upload_csr: neighbour-label profile kernel failed");
+      }
+    }
+  }
   *out = g;
   return FSIG_OK;
 }
@@ -303,7 +346,7 @@ void fsig_graph_free(fsig_ctx* c, fsig_graph* g) {
   if (!g) return;
   if (c) (void)hipSetDevice(c->device);
   void* ptrs[] = {g->d_off, g->d_adj, g->d_labels, g->d_inv_index, g->d_inv_list,
-                  g->d_groups, g->d_pedges, g->d_units, g->d_sig};
+                  g->d_groups, g->d_pedges, g->d_units, g->d_sig, g->d_nlf};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete g;
@@ -415,6 +458,8 @@ int fsig_set_tune(fsig_ctx* c, const fsig_tune* t) {
   const uint32_t g = t->lanes_per_row;
   if (g != 0 && g != 4 && g != 8 && g != 16 && g != 32 && g != 64) return set_err(c, FSIG_EINVAL, "tune: lanes_per_row");
   if (t->strategy > 2) return set_err(c, FSIG_EINVAL, "tune: strategy");
+  if (t->fused_tail > 1) return set_err(c, FSIG_EINVAL, "tune: fused_tail");
+  if (t->nlf_filter > 1) return set_err(c, FSIG_EINVAL, "tune: nlf_filter");
   c->tune = *t;
   if (c->tune.refine_passes == 0) c->tune.refine_passes = 4;
   return FSIG_OK;

@@ -32,7 +32,23 @@ struct LabelSegs {
   uint32_t flat[MAXQ + 1];
   uint32_t cmask[MAXQ];
   uint32_t deg[MAXQ];  // per class
+  // neighbour-label filter (nlf null: off): class c needs, for each of its
+  // nreq[c] entries e = req[c * MAXQ + i], nlf[v * L + (e >> 8)] >= (e & 255)
+  uint32_t nreq[MAXQ];
+  const uint32_t* req;
+  const uint8_t* nlf;
+  uint32_t L;
 };
+
+__device__ __forceinline__ bool nlf_ok(const uint8_t* __restrict__ nlf, uint32_t L, uint32_t v,
+                                       const uint32_t* __restrict__ req, uint32_t nreq) {
+  const uint8_t* row = nlf + (uint64_t)v * L;
+  for (uint32_t i = 0; i < nreq; i++) {
+    const uint32_t e = req[i];
+    if (row[e >> 8] < (e & 255u)) return false;
+  }
+  return true;
+}
 
 __global__ void __launch_bounds__(FSIG_BLOCK) k_ldf(LabelSegs S, const uint32_t* __restrict__ inv_list,
                                                     const uint32_t* __restrict__ off, uint32_t* __restrict__ mask) {
@@ -46,7 +62,7 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_ldf(LabelSegs S, const uint32_t*
     while (cm) {
       const uint32_t c = __builtin_ctz(cm);
       cm &= cm - 1;
-      if (d >= S.deg[c]) m |= 1u << c;
+      if (d >= S.deg[c] && (!S.nlf || nlf_ok(S.nlf, S.L, v, S.req + c * MAXQ, S.nreq[c]))) m |= 1u << c;
     }
     mask[v] = m;
   }
@@ -300,6 +316,15 @@ int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_
     LS.cmask[s] |= 1u << a;
     LS.deg[a] = P.class_deg[a];
     seg_of_class[a] = s;
+  }
+  if (g->d_nlf && c->tune.nlf_filter == 0) {
+    uint32_t* dreq = (uint32_t*)A.alloc(4ull * MAXQ * MAXQ);
+    if (!dreq) return set_err(c, FSIG_ENOMEM, "filter: arena exhausted");
+    FSIG_HIP(c, hipMemcpyAsync(dreq, P.nlf_req, 4ull * MAXQ * MAXQ, hipMemcpyHostToDevice, c->stream));
+    for (uint32_t a = 0; a < nc; a++) LS.nreq[a] = P.nlf_n[a];
+    LS.req = dreq;
+    LS.nlf = g->d_nlf;
+    LS.L = g->num_labels;
   }
   const uint64_t ldf_items = LS.flat[LS.n];
   if (ldf_items)

@@ -64,7 +64,7 @@ struct fsig_ctx {
   // resolved after the stream drains (no synchronisation per launch).
   std::vector<TimedLaunch> timed;
   size_t timed_used = 0;
-  fsig_tune tune{0, 0, 0, 4};
+  fsig_tune tune{0, 0, 0, 4, 0, 0};
   std::vector<uint32_t> last_cand;
   uint64_t last_peak = 0;
   // device counters (64 words): [0..7] per-launch results, [16 + 4f ..] per
@@ -98,6 +98,9 @@ struct fsig_graph {
   uint32_t* d_inv_index = nullptr;
   uint32_t* d_inv_list = nullptr;
   std::vector<uint32_t> h_inv_index;
+  // neighbour-label profile: nlf[v * num_labels + l] = number of neighbours
+  // of v with label l, saturated at 255 (null for one label or > 255 labels)
+  uint8_t* d_nlf = nullptr;
   // PCSR and signatures (alg 2)
   bool has_pcsr = false;
   int ref_unit_ids = 0;
@@ -125,6 +128,10 @@ struct QueryPlan {
   uint32_t class_sig[MAXQ][16];  // alg 2: query signature of a class
   uint32_t order[MAXQ];        // join position -> query vertex
   uint32_t pos[MAXQ];          // query vertex -> join position
+  // neighbour-label requirements of a class: (label << 8) | count, one entry
+  // per label among the representative's neighbours
+  uint32_t nlf_n[MAXQ];
+  uint32_t nlf_req[MAXQ][MAXQ];
 };
 
 int plan_classes(QueryPlan& P, uint32_t k, const int32_t* qlabels, const uint32_t* qoff,

@@ -25,6 +25,7 @@
 #pragma once
 
 #include <algorithm>
+#include <type_traits>
 
 #include "device.h"
 #include "internal.h"
@@ -161,7 +162,10 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join(JoinLevel L, Src S, JoinIO 
     }
     uint64_t base = 0;
     uint64_t child0 = 0;
-    if (WRITE) child0 = io.off[r];
+    if (WRITE) {
+      child0 = io.off[r];
+      if (io.off[r + 1] == child0) continue;  // no children (or counted by a fused tail)
+    }
     const uint32_t* dl = L.nb ? S.list(d) + sh_s[gw][d] : nullptr;
     // Count pass with one backward list of candidates only: the children are
     // the list minus the earlier vertices of the same label that occur in
@@ -252,6 +256,344 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join(JoinLevel L, Src S, JoinIO 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused count of the last two levels (count mode, candidate-edge lists).
+//
+// For a parent row at level p (p columns) the children are the candidates x
+// of join position p (level L1) and each child's leaves are the candidates y
+// of the last position (level L2).  L2's backward lists split into the ones
+// of parent columns, which all children share, and at most one of column p
+// (the child's own).  The shared intersection I (minus the leaves ruled out
+// by parent-column conflicts) is built once per parent in LDS; a child's leaf
+// count is then |I ∩ own(x)| (or |I| minus x's own image when w is not
+// adjacent to position p).  When I does not fit the tile, every child is
+// counted by the group against all lists; with no shared list the count is
+// |own(x)| minus the parent-column conflicts found in it.  Level p+1 is never
+// written.
+constexpr uint32_t kNone = 0xffffffffu;
+
+template <int G>
+__device__ __forceinline__ uint64_t group_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+
+struct TailLds {
+  uint32_t* s1;
+  uint32_t* n1;
+  uint32_t* cf1;
+  uint32_t* s2;
+  uint32_t* n2;
+  uint32_t* cf2;
+  uint32_t* slot;  // G words: children counted by the group
+  uint32_t* tile;  // tw words
+  uint32_t tw;
+};
+
+__device__ __forceinline__ void lds_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Leaves of one parent row; returns this lane's share of the leaf count
+// (sum over the group's lanes = the row's leaves) and sets `children` (group
+// uniform) to the number of children x.
+template <int G>
+__device__ __forceinline__ uint64_t tail_row(const JoinLevel& L1, const CandEdgeSrc& S1, const JoinLevel& L2,
+                                             const CandEdgeSrc& S2, const uint32_t (&rv)[(MAXQ + G - 1) / G],
+                                             const TailLds& sh, uint32_t gl, uint64_t gmask, uint64_t below,
+                                             uint64_t& children, bool& defer, unsigned long long& bytes) {
+  const uint32_t p = L1.p;
+  defer = false;
+  uint32_t dlen = L1.wsize, d = 0;
+  uint64_t el = 0;
+  for (uint32_t i = 0; i < L1.nb; i++) {
+    const uint32_t x = group_col<G>(rv, L1.bcol[i]);
+    const uint32_t s = S1.off[i][x], n = S1.off[i][x + 1] - s;
+    sh.s1[i] = s;
+    sh.n1[i] = n;
+    el += n;
+    if (i == 0 || n < dlen) { dlen = n; d = i; }
+  }
+  for (uint32_t j = 0; j < L1.ncf; j++) {
+    uint32_t x = group_col<G>(rv, L1.cfcol[j]);
+    if (!((L1.cfsame >> j) & 1u)) x = L1.cfcand[j][x];
+    sh.cf1[j] = x;
+  }
+  // leaf lists of parent columns (shared) and of column p (own)
+  uint32_t ns = 0, own = kNone, ds = 0, dslen = kNone;
+  for (uint32_t i = 0; i < L2.nb; i++) {
+    const uint32_t c = L2.bcol[i];
+    if (c == p) { own = i; continue; }
+    const uint32_t x = group_col<G>(rv, c);
+    const uint32_t s = S2.off[i][x], n = S2.off[i][x + 1] - s;
+    sh.s2[i] = s;
+    sh.n2[i] = n;
+    el += n;
+    if (n < dslen) { dslen = n; ds = i; }
+    ns++;
+  }
+  // leaf conflicts with parent columns, as candidate indices of the leaf
+  // class (kNone: that vertex is no candidate, nothing to exclude); ccf is the
+  // conflict with column p, resolved per child
+  uint32_t ccf = kNone;
+  for (uint32_t j = 0; j < L2.ncf; j++) {
+    const uint32_t c = L2.cfcol[j];
+    if (c == p) { ccf = j; sh.cf2[j] = kNone; continue; }
+    uint32_t t = group_col<G>(rv, c);
+    if (!((L2.cfsame >> j) & 1u)) {
+      const uint32_t id = L2.cfcand[j][t];
+      const uint32_t at = sorted_index(L2.wcand, L2.wsize, id);
+      t = (at < L2.wsize && L2.wcand[at] == id) ? at : kNone;
+    }
+    sh.cf2[j] = t;
+  }
+  uint64_t lbytes = gl == 0 ? 4ull * p + 8ull * (L1.nb + ns) + 4ull * el : 0;
+  const uint32_t* dl = L1.nb ? S1.lst[d] + sh.s1[d] : nullptr;
+  if (own == kNone) {
+    // No own list: every child has the same leaves I except its own image.
+    // |I| once per parent (one shared list: its length minus the conflicts
+    // found in it; more: a counting walk), then one membership test per child.
+    uint64_t isz = 0;
+    const uint32_t* dl2 = S2.lst[ds] + sh.s2[ds];
+    if (ns == 1) {
+      uint32_t hit = 0;
+      for (uint32_t j0 = 0; j0 < L2.ncf; j0 += G) {
+        const uint32_t j = j0 + gl;
+        const bool in = j < L2.ncf && sh.cf2[j] != kNone && sorted_contains(dl2, dslen, sh.cf2[j]);
+        hit += (uint32_t)__popcll(__ballot(in) & gmask);
+      }
+      isz = dslen - hit;
+      if (gl == 0) lbytes -= 4ull * dslen;  // the list itself is not read
+    } else {
+      for (uint32_t t0 = 0; t0 < dslen; t0 += G) {
+        const uint32_t t = t0 + gl;
+        bool ok = t < dslen;
+        const uint32_t y = ok ? dl2[t] : 0u;
+        for (uint32_t i = 0; i < L2.nb && ok; i++) {
+          if (i == ds) continue;
+          ok = sorted_contains(S2.lst[i] + sh.s2[i], sh.n2[i], y);
+        }
+        for (uint32_t j = 0; j < L2.ncf && ok; j++) ok = y != sh.cf2[j];
+        isz += (uint32_t)__popcll(__ballot(ok) & gmask);
+      }
+    }
+    uint64_t kids = 0, leaves = 0;
+    for (uint32_t t0 = 0; t0 < dlen; t0 += G) {
+      const uint32_t t = t0 + gl;
+      bool ok = t < dlen;
+      uint32_t x = 0;
+      if (ok) x = L1.nb ? dl[t] : t;
+      for (uint32_t i = 0; i < L1.nb && ok; i++) {
+        if (i == d) continue;
+        ok = sorted_contains(S1.lst[i] + sh.s1[i], sh.n1[i], x);
+      }
+      if (ok && L1.ncf) {
+        const uint32_t xid = L1.any_cross ? L1.wcand[x] : x;
+        for (uint32_t j = 0; j < L1.ncf && ok; j++) ok = (((L1.cfsame >> j) & 1u) ? x : xid) != sh.cf1[j];
+      }
+      kids += __popcll(__ballot(ok) & gmask);
+      if (ok) {
+        uint64_t cnt = isz;
+        if (ccf != kNone) {  // x's image is a leaf candidate unless some shared list lacks it
+          uint32_t cx = x;
+          if (!((L2.cfsame >> ccf) & 1u)) {
+            const uint32_t id = L2.cfcand[ccf][x];
+            const uint32_t at = sorted_index(L2.wcand, L2.wsize, id);
+            cx = (at < L2.wsize && L2.wcand[at] == id) ? at : kNone;
+          }
+          bool in = cx != kNone;
+          for (uint32_t i = 0; i < L2.nb && in; i++) in = sorted_contains(S2.lst[i] + sh.s2[i], sh.n2[i], cx);
+          if (in) cnt--;
+          lbytes += 4ull * ns * (32 - __builtin_clz(dslen | 1));
+        }
+        leaves += cnt;
+      }
+    }
+    children = kids;
+    bytes += lbytes;
+    return leaves;
+  }
+  // A row whose shared intersection does not fit the tile, or that may have
+  // many children while I is large, is deferred: its children are counted here and written by the caller,
+  // which counts their leaves one row per group.  Other rows stage I in LDS once and count each
+  // child's leaves cooperatively: lanes split the shorter of I and the
+  // child's own list and search the other.
+  bool defer_row = false;
+  uint32_t isz = 0;
+  if (ns > 0) {
+    const uint32_t* dl2 = S2.lst[ds] + sh.s2[ds];
+    for (uint32_t t0 = 0; t0 < dslen; t0 += G) {
+      const uint32_t t = t0 + gl;
+      bool ok = t < dslen;
+      const uint32_t y = ok ? dl2[t] : 0u;
+      for (uint32_t i = 0; i < L2.nb && ok; i++) {
+        if (i == ds || i == own) continue;
+        ok = sorted_contains(S2.lst[i] + sh.s2[i], sh.n2[i], y);
+      }
+      for (uint32_t j = 0; j < L2.ncf && ok; j++) ok = y != sh.cf2[j];
+      const uint64_t bal = __ballot(ok) & gmask;
+      const uint32_t nb = (uint32_t)__popcll(bal);
+      if (isz + nb > sh.tw) { defer_row = true; break; }  // I does not fit
+      if (ok) sh.tile[isz + __popcll(bal & below)] = y;
+      isz += nb;
+    }
+    lds_wave_sync();
+    // many children, each with a large I to intersect: spread them instead
+    defer_row = defer_row || (dlen > 4u * G && isz > 2u * G);
+  }
+  uint64_t kids = 0, leaves = 0;
+  for (uint32_t t0 = 0; t0 < dlen; t0 += G) {
+    const uint32_t t = t0 + gl;
+    bool ok = t < dlen;
+    uint32_t x = 0;
+    if (ok) x = L1.nb ? dl[t] : t;
+    for (uint32_t i = 0; i < L1.nb && ok; i++) {
+      if (i == d) continue;
+      ok = sorted_contains(S1.lst[i] + sh.s1[i], sh.n1[i], x);
+    }
+    if (ok && L1.ncf) {
+      const uint32_t xid = L1.any_cross ? L1.wcand[x] : x;
+      for (uint32_t j = 0; j < L1.ncf && ok; j++) ok = (((L1.cfsame >> j) & 1u) ? x : xid) != sh.cf1[j];
+    }
+    const uint64_t bal = __ballot(ok) & gmask;
+    kids += __popcll(bal);
+    if (defer_row) continue;
+    if (ns == 0) {
+      // own list only: its length minus the parent-column conflicts in it
+      if (ok) {
+        const uint32_t s = S2.off[own][x], n = S2.off[own][x + 1] - s;
+        const uint32_t* l = S2.lst[own] + s;
+        uint64_t cnt = n;
+        for (uint32_t j = 0; j < L2.ncf; j++)
+          if (sh.cf2[j] != kNone && sorted_contains(l, n, sh.cf2[j])) cnt--;
+        leaves += cnt;
+        lbytes += 8 + 4ull * L2.ncf * (32 - __builtin_clz(n | 1));
+      }
+      continue;
+    }
+    if (ok) sh.slot[__popcll(bal & below)] = x;
+    lds_wave_sync();
+    const uint32_t nsurv = (uint32_t)__popcll(bal);
+    for (uint32_t q = 0; q < nsurv; q++) {
+      const uint32_t xx = sh.slot[q];
+      const uint32_t s = S2.off[own][xx], n = S2.off[own][xx + 1] - s;
+      const uint32_t* l = S2.lst[own] + s;
+      uint32_t got = 0;
+      if (isz <= n) {
+        for (uint32_t e0 = 0; e0 < isz; e0 += G) {
+          const uint32_t e = e0 + gl;
+          const bool in = e < isz && sorted_contains(l, n, sh.tile[e]);
+          got += (uint32_t)__popcll(__ballot(in) & gmask);
+        }
+        if (gl == 0) lbytes += 8 + 4ull * isz * (32 - __builtin_clz(n | 1));
+      } else {
+        for (uint32_t e0 = 0; e0 < n; e0 += G) {
+          const uint32_t e = e0 + gl;
+          const bool in = e < n && sorted_contains(sh.tile, isz, l[e]);
+          got += (uint32_t)__popcll(__ballot(in) & gmask);
+        }
+        if (gl == 0) lbytes += 8 + 4ull * n;
+      }
+      if (gl == 0) leaves += got;
+    }
+    lds_wave_sync();
+  }
+  defer = defer_row;
+  children = kids;
+  bytes += lbytes;
+  return leaves;
+}
+
+template <int G>
+struct TailShared {
+  static constexpr int GPB = FSIG_BLOCK / G;
+  static constexpr uint32_t kTileWords = 4096 / GPB;  // 16 KiB of tiles per block
+  uint32_t s1[GPB][MAXQ], n1[GPB][MAXQ], cf1[GPB][MAXQ];
+  uint32_t s2[GPB][MAXQ], n2[GPB][MAXQ], cf2[GPB][MAXQ];
+  uint32_t slot[GPB][G];
+  uint32_t tile[GPB][kTileWords];
+  __device__ TailLds at(uint32_t gw) {
+    return TailLds{s1[gw], n1[gw], cf1[gw], s2[gw], n2[gw], cf2[gw], slot[gw], tile[gw], kTileWords};
+  }
+};
+
+struct TailIO {
+  const uint32_t* rows;
+  uint64_t nrows;
+  uint32_t* cnt;              // per row: children of a deferred row, else 0
+  unsigned long long* res;    // [0] += leaves, [1] += children of counted rows
+  unsigned long long* stats;  // [0] += algorithmic bytes, [1] += parent rows
+  uint32_t tile_cap;          // words of the shared-intersection tile (0: all)
+};
+
+template <int G>
+__global__ void __launch_bounds__(FSIG_BLOCK) k_join_tail(JoinLevel L1, CandEdgeSrc S1, JoinLevel L2,
+                                                          CandEdgeSrc S2, TailIO io) {
+  constexpr int GPB = FSIG_BLOCK / G;
+  constexpr int NR = (MAXQ + G - 1) / G;
+  __shared__ TailShared<G> sh;
+  const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
+  const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
+  const uint64_t below = (1ull << lane) - 1;
+  const uint64_t ngroups = (uint64_t)gridDim.x * GPB;
+  TailLds t = sh.at(gw);
+  if (io.tile_cap && io.tile_cap < t.tw) t.tw = io.tile_cap;
+  unsigned long long my_leaves = 0, my_kids = 0, my_bytes = 0, my_rows = 0;
+  for (uint64_t r = (uint64_t)xcd_block() * GPB + gw; r < io.nrows; r += ngroups) {
+    uint32_t rv[NR];
+    const uint32_t* row = io.rows + r * L1.p;
+#pragma unroll
+    for (int j = 0; j < NR; j++) {
+      const uint32_t col = j * G + gl;
+      rv[j] = col < L1.p ? row[col] : 0u;
+    }
+    uint64_t kids = 0;
+    bool defer = false;
+    my_leaves += tail_row<G>(L1, S1, L2, S2, rv, t, gl, gmask, below, kids, defer, my_bytes);
+    if (gl == 0) {
+      io.cnt[r] = defer ? (uint32_t)kids : 0u;
+      if (!defer) my_kids += kids;
+      my_rows += 1;
+    }
+  }
+  my_leaves = wave_sum_u64(my_leaves);
+  my_kids = wave_sum_u64(my_kids);
+  my_bytes = wave_sum_u64(my_bytes);
+  my_rows = wave_sum_u64(my_rows);
+  if (lane == 0) {
+    atomicAdd(&io.res[0], my_leaves);
+    atomicAdd(&io.res[1], my_kids);
+    if (io.stats) {
+      atomicAdd(&io.stats[0], my_bytes);
+      atomicAdd(&io.stats[1], my_rows);
+    }
+  }
+}
+
+// Whether the fused tail applies to leaf level L2 (it needs a backward list).
+inline bool tail_applies(const JoinLevel& L2) { return L2.nb >= 1; }
+
+inline int launch_join_tail(fsig_ctx* c, uint32_t G, const JoinLevel& L1, const CandEdgeSrc& S1,
+                            const JoinLevel& L2, const CandEdgeSrc& S2, const TailIO& io) {
+  if (io.nrows == 0) return FSIG_OK;
+  const uint64_t gpb = FSIG_BLOCK / G;
+  uint64_t blocks = std::min<uint64_t>((io.nrows + gpb - 1) / gpb, (uint64_t)c->num_cus * 16);
+  if (blocks >= 64) blocks &= ~7ull;
+  const uint32_t grid = (uint32_t)std::max<uint64_t>(blocks, 1);
+  switch (G) {
+    case 4: k_join_tail<4><<<grid, FSIG_BLOCK, 0, c->stream>>>(L1, S1, L2, S2, io); break;
+    case 8: k_join_tail<8><<<grid, FSIG_BLOCK, 0, c->stream>>>(L1, S1, L2, S2, io); break;
+    case 16: k_join_tail<16><<<grid, FSIG_BLOCK, 0, c->stream>>>(L1, S1, L2, S2, io); break;
+    case 32: k_join_tail<32><<<grid, FSIG_BLOCK, 0, c->stream>>>(L1, S1, L2, S2, io); break;
+    default: k_join_tail<64><<<grid, FSIG_BLOCK, 0, c->stream>>>(L1, S1, L2, S2, io); break;
+  }
+  FSIG_HIP(c, hipGetLastError());
+  return FSIG_OK;
+}
+
 // Parent rows of the child window [lo, hi): out[0] = last r with off[r] <= lo,
 // out[1] = first r with off[r] >= hi (off has nrows+1 entries).
 static __global__ void k_window_rows(const uint64_t* __restrict__ off, uint64_t nrows, uint64_t lo, uint64_t hi,
@@ -334,6 +676,9 @@ class JoinDriver {
     const bool last = p + 1 == k_;
     unsigned long long* res = c_->d_counters;  // [0] children of this launch
     int rc;
+    bool fused = false;
+    if constexpr (std::is_same<Src, CandEdgeSrc>::value)
+      fused = mode_ == FSIG_MODE_COUNT && p + 2 == k_ && c_->tune.fused_tail == 0 && tail_applies(pv_.level(p + 1));
     if (last && mode_ == FSIG_MODE_COUNT) {
       FSIG_HIP(c_, hipMemsetAsync(res, 0, sizeof(unsigned long long), c_->stream));
       JoinIO io{rows, 0, R, nullptr, nullptr, nullptr, 0, 0, res, family_counters(c_, FSIG_K_JOIN_COUNT)};
@@ -348,7 +693,14 @@ class JoinDriver {
     uint32_t* cnt = (uint32_t*)c_->arena.alloc(4 * R);
     uint64_t* off = (uint64_t*)c_->arena.alloc(8 * (R + 1));
     if (!cnt || !off) return set_err(c_, FSIG_ENOMEM, "join: arena exhausted for row counts");
-    {
+    if constexpr (std::is_same<Src, CandEdgeSrc>::value) {
+      if (fused) {
+        // rows whose leaves the tail counts get 0 children; deferred rows
+        // keep theirs and go through the write pass and the last level
+        if ((rc = count_tail(rows, R, cnt, L, S, pv_.level(p + 1), pv_.src(p + 1)))) return rc;
+      }
+    }
+    if (!fused) {
       JoinIO io{rows, 0, R, cnt, nullptr, nullptr, 0, 0, nullptr, family_counters(c_, FSIG_K_JOIN_COUNT)};
       KTimer t(c_, FSIG_K_JOIN_COUNT);
       if ((rc = launch_join<false>(c_, G_, L, S, io))) return rc;
@@ -395,6 +747,27 @@ class JoinDriver {
       c_->arena.release(m2);
     }
     c_->arena.release(mk);
+    return FSIG_OK;
+  }
+
+  // The last two levels in one pass (count mode): leaves of the counted rows
+  // are added to the total and their children count towards the work limit;
+  // deferred rows keep their children in cnt.
+  int count_tail(const uint32_t* rows, uint64_t R, uint32_t* cnt, const JoinLevel& L1, const CandEdgeSrc& S1,
+                 const JoinLevel& L2, const CandEdgeSrc& S2) {
+    unsigned long long* res = c_->d_counters;
+    FSIG_HIP(c_, hipMemsetAsync(res, 0, 2 * sizeof(unsigned long long), c_->stream));
+    const TailIO io{rows, R, cnt, res, family_counters(c_, FSIG_K_JOIN_COUNT), c_->tune.lds_tile};
+    KTimer t(c_, FSIG_K_JOIN_COUNT);
+    int rc = launch_join_tail(c_, std::max<uint32_t>(G_, 8), L1, S1, L2, S2, io);
+    if (rc) return rc;
+    t.done(0, 0);
+    uint64_t h[2];
+    if ((rc = read_u64(c_, res, h, 2))) return rc;
+    rows_ += h[1];
+    const uint64_t limit = c_->work_limit;
+    if (limit && rows_ > limit) return set_err(c_, FSIG_ELIMIT, "join: work limit exceeded");
+    total_ += std::min(h[0], remaining());
     return FSIG_OK;
   }
 

@@ -109,6 +109,21 @@ int plan_classes(QueryPlan& P, uint32_t k, const int32_t* qlabels, const uint32_
     for (uint32_t w = 0; w < k; w++)
       if ((P.adjm[u] >> w) & 1) P.thr[a][P.cls[w]]++;
   }
+  // neighbour-label frequency: an injective image of u needs, for every
+  // label l, at least as many label-l neighbours as u has
+  for (uint32_t a = 0; a < P.nclass; a++) {
+    const uint32_t u = P.rep[a];
+    uint32_t lab[MAXQ], cnt[MAXQ], m = 0;
+    for (uint32_t w = 0; w < k; w++) {
+      if (!((P.adjm[u] >> w) & 1)) continue;
+      uint32_t j = 0;
+      while (j < m && lab[j] != (uint32_t)P.label[w]) j++;
+      if (j == m) { lab[m] = (uint32_t)P.label[w]; cnt[m] = 0; m++; }
+      cnt[j]++;
+    }
+    P.nlf_n[a] = m;
+    for (uint32_t j = 0; j < m; j++) P.nlf_req[a][j] = (lab[j] << 8) | cnt[j];
+  }
   return FSIG_OK;
 }
 

@@ -145,17 +145,26 @@ int fsig_last_candidates(const fsig_ctx* ctx, uint32_t k, uint32_t* sizes);
 /* Peak arena bytes of the last fsig_match. */
 uint64_t fsig_last_peak_bytes(const fsig_ctx* ctx);
 
-/* Join tuning (alg 1 and 2), all fields set at once; 0 selects the automatic
- * choice.  lanes_per_row in {4,8,16,32,64}: lanes of a wavefront that share
- * one partial embedding (auto: the power of two at or above the mean list
- * length).  strategy: 0 auto, 1 binary search of the other lists in global
- * memory, 2 stage the other lists in LDS when they fit (lds_tile words per
- * row).  refine_passes: cap on refinement passes (0 = 4). */
+/* Join tuning, all fields set at once; 0 selects the automatic choice.
+ * lanes_per_row in {4,8,16,32,64}: lanes of a wavefront that share one
+ * partial embedding (auto: the power of two at or above the mean list
+ * length).  strategy (fsig_match_batch only): 0 lockstep join, 1 persistent
+ * walk, 2 persistent walk with the second list staged in LDS (lds_tile words
+ * per group).  For fsig_match, lds_tile caps the LDS tile of the fused tail
+ * (words per lane group).  refine_passes: cap on refinement passes (0 = 4). */
 typedef struct fsig_tune {
   uint32_t lanes_per_row;
   uint32_t strategy;
   uint32_t lds_tile;
   uint32_t refine_passes; /* cap on refinement passes (default 4) */
+  uint32_t fused_tail;    /* 0: count the last two levels of a count-mode
+                             alg-1 match in one pass (the next-to-last level
+                             is never written) when the last query vertex has
+                             a backward neighbour; 1: level by level */
+  uint32_t nlf_filter;    /* 0: alg-1 filtering also requires, per label, at
+                             least as many neighbours of that label as the
+                             query vertex has (graphs with 2..255 labels);
+                             1: label and degree only, as the reference */
 } fsig_tune;
 int fsig_set_tune(fsig_ctx* ctx, const fsig_tune* t);
 int fsig_get_tune(const fsig_ctx* ctx, fsig_tune* t);

@@ -98,12 +98,42 @@ def test_candidate_sets_equal_oracle_refinement(gpu_ctx):
     off, adj, lab = g.arrays()
     dg = gpu_ctx.upload(g)
     passes = gpu_ctx.get_tune()["refine_passes"]
-    for s in range(5):
-        q = g.walk_query(6, 5, 12, seed=500 + s)
-        F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT)
-        got = gpu_ctx.last_candidates(q.k)
-        want = O.refine(q.labels, q.adjm(), off, adj, lab, passes)
-        assert np.array_equal(got, want)
+    gpu_ctx.set_tune(nlf_filter=1)  # the reference's filters only
+    try:
+        for s in range(5):
+            q = g.walk_query(6, 5, 12, seed=500 + s)
+            F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT)
+            got = gpu_ctx.last_candidates(q.k)
+            want = O.refine(q.labels, q.adjm(), off, adj, lab, passes)
+            assert np.array_equal(got, want)
+    finally:
+        gpu_ctx.set_tune()
+
+
+def test_neighbour_label_filter_is_sound_and_prunes(gpu_ctx):
+    # u (label 0) has two label-1 neighbours in different classes (a leaf and
+    # a vertex with a further neighbour): refinement asks one neighbour in
+    # each class, the neighbour-label filter two label-1 neighbours
+    g = F.HostGraph.chung_lu(8000, 30000, 2.5, 33).set_labels("uniform", 3, seed=34)
+    off, adj, lab = g.arrays()
+    dg = gpu_ctx.upload(g)
+    queries = [F.Query(4, [0, 1, 1, 2], [(0, 1), (0, 2), (2, 3)])]
+    queries += [g.walk_query(6, 5, 12, seed=900 + s) for s in range(4)]
+    pruned = False
+    for q in queries:
+        want, _ = O.match(q.labels, q.adjm(), off, adj, lab, threads=8)
+        sizes = {}
+        for flag in (0, 1):
+            gpu_ctx.set_tune(nlf_filter=flag)
+            try:
+                assert F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT).count == want
+                sizes[flag] = gpu_ctx.last_candidates(q.k)
+            finally:
+                gpu_ctx.set_tune()
+        assert np.all(sizes[0] <= sizes[1])
+        pruned |= bool(np.any(sizes[0] < sizes[1]))
+    assert pruned
+    dg.free()
 
 
 def test_gsi_candidate_sets_equal_oracle_signature_filter(gpu_ctx):
