@@ -298,6 +298,134 @@ __global__ void k_bpair_base(const uint64_t* __restrict__ gscan, const BPair* __
   if (i < np) out[i] = gscan[P[i].flat];
 }
 
+// One (query, class a) with every target class b of a pair a -> b: the
+// candidate-edge lists of all of a's pairs come from one sweep over each
+// candidate's adjacency (one mask gather per neighbour serves every target).
+struct BSrc {
+  const uint32_t* acand;
+  const uint32_t* mask;
+  uint32_t asize;
+  uint32_t tmask;
+  uint64_t flat;
+  uint32_t pidx[MAXQ];  // pair index of target b
+};
+
+template <int G>
+__global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_count_multi(const BSrc* __restrict__ S, uint32_t ns,
+                                                                   uint64_t total, const BPair* __restrict__ P,
+                                                                   const uint32_t* __restrict__ off,
+                                                                   const uint32_t* __restrict__ adj,
+                                                                   uint32_t* __restrict__ cnt) {
+  constexpr int GPB = FSIG_BLOCK / G;
+  __shared__ uint32_t acc[GPB][MAXQ];
+  const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
+  const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
+  const uint64_t ngroups = (uint64_t)gridDim.x * GPB;
+  for (uint64_t f = (uint64_t)xcd_block() * GPB + gw; f < total; f += ngroups) {
+    const BSrc& sc = S[find_flat(S, ns, f)];
+    const uint32_t i0 = (uint32_t)(f - sc.flat);
+    const uint32_t v = sc.acand[i0];
+    const uint32_t tm = sc.tmask;
+    if (gl == 0)
+      for (uint32_t t = tm; t; t &= t - 1) acc[gw][__builtin_ctz(t)] = 0;
+    const uint32_t s = off[v], e = off[v + 1];
+    for (uint32_t j = s; j < e; j += G) {
+      const uint32_t i = j + gl;
+      const uint32_t m = i < e ? (sc.mask[adj[i]] & tm) : 0u;
+      for (uint32_t t = tm; t; t &= t - 1) {
+        const uint32_t b = __builtin_ctz(t);
+        const uint32_t got = (uint32_t)__popcll(__ballot((m >> b) & 1u) & gmask);
+        if (gl == 0) acc[gw][b] += got;
+      }
+    }
+    if (gl == 0)
+      for (uint32_t t = tm; t; t &= t - 1) {
+        const uint32_t b = __builtin_ctz(t);
+        cnt[P[sc.pidx[b]].flat + i0] = acc[gw][b];
+      }
+  }
+}
+
+template <int G>
+__global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_fill_multi(const BSrc* __restrict__ S, uint32_t ns,
+                                                                  uint64_t total, const BPair* __restrict__ P,
+                                                                  const uint32_t* __restrict__ off,
+                                                                  const uint32_t* __restrict__ adj,
+                                                                  const uint64_t* __restrict__ gscan,
+                                                                  uint32_t* __restrict__ lst_all) {
+  constexpr int GPB = FSIG_BLOCK / G;
+  __shared__ uint64_t at[GPB][MAXQ];
+  const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
+  const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
+  const uint64_t below = (1ull << lane) - 1;
+  const uint64_t ngroups = (uint64_t)gridDim.x * GPB;
+  for (uint64_t f = (uint64_t)xcd_block() * GPB + gw; f < total; f += ngroups) {
+    const BSrc& sc = S[find_flat(S, ns, f)];
+    const uint32_t i0 = (uint32_t)(f - sc.flat);
+    const uint32_t v = sc.acand[i0];
+    const uint32_t tm = sc.tmask;
+    if (gl == 0)
+      for (uint32_t t = tm; t; t &= t - 1) {
+        const uint32_t b = __builtin_ctz(t);
+        const BPair& pd = P[sc.pidx[b]];
+        const uint64_t pbase = gscan[pd.flat];
+        const uint64_t a0 = gscan[pd.flat + i0];
+        at[gw][b] = a0;
+        pd.off_out[i0] = (uint32_t)(a0 - pbase);
+        if (i0 + 1 == pd.asize) pd.off_out[pd.asize] = (uint32_t)(gscan[pd.flat + i0 + 1] - pbase);
+      }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t s = off[v], e = off[v + 1];
+    for (uint32_t j = s; j < e; j += G) {
+      const uint32_t i = j + gl;
+      uint32_t x = 0, m = 0;
+      if (i < e) {
+        x = adj[i];
+        m = sc.mask[x] & tm;
+      }
+      for (uint32_t t = tm; t; t &= t - 1) {
+        const uint32_t b = __builtin_ctz(t);
+        const bool hit = (m >> b) & 1u;
+        const uint64_t bal = __ballot(hit) & gmask;
+        if (bal == 0) continue;
+        const uint64_t a0 = at[gw][b];
+        if (hit) {
+          const BPair& pd = P[sc.pidx[b]];
+          lst_all[a0 + __popcll(bal & below)] = pd.bpos ? pd.bpos[x] : sorted_index(pd.bcand, pd.bsize, x);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (gl == 0) at[gw][b] = a0 + __popcll(bal);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    }
+  }
+}
+
+// Longest candidate-edge list of each pair (one block per pair), from the
+// list offsets.
+__global__ void __launch_bounds__(FSIG_BLOCK) k_bpair_maxlen(const BPair* __restrict__ P,
+                                                             const uint64_t* __restrict__ gscan,
+                                                             unsigned int* __restrict__ maxlen) {
+  __shared__ uint32_t red[FSIG_BLOCK];
+  const BPair& pd = P[blockIdx.x];
+  uint32_t mx = 0;
+  for (uint32_t i = threadIdx.x; i < pd.asize; i += FSIG_BLOCK)
+    mx = max(mx, (uint32_t)(gscan[pd.flat + i + 1] - gscan[pd.flat + i]));
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (uint32_t o = FSIG_BLOCK / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) maxlen[blockIdx.x] = red[0];
+}
+
 // ---------------------------------------------------------------------------
 // Persistent depth-first join.
 struct BLevel {
@@ -792,7 +920,8 @@ struct PlainShared {
 template <int G, bool WRITE>
 __global__ void __launch_bounds__(FSIG_BLOCK) k_join_multi(const MLevel* __restrict__ ML,
                                                            const uint64_t* __restrict__ qbase, uint32_t nlive,
-                                                           const uint32_t* __restrict__ rows, uint64_t nrows,
+                                                           const uint32_t* __restrict__ rows, uint64_t row_lo,
+                                                           uint64_t row_hi,
                                                            uint32_t p, uint32_t* __restrict__ cnt,
                                                            const uint64_t* __restrict__ off,
                                                            uint32_t* __restrict__ out,
@@ -814,7 +943,7 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join_multi(const MLevel* __restr
   unsigned long long my_bytes = 0, my_units = 0;
   uint32_t cur_q = 0xffffffffu;
   unsigned long long acc = 0;
-  for (uint64_t r = (uint64_t)xcd_block() * GPB + gw; r < nrows; r += ngroups) {
+  for (uint64_t r = row_lo + (uint64_t)xcd_block() * GPB + gw; r < row_hi; r += ngroups) {
     uint32_t lo = 0, hi = nlive;  // query of the row: last j with qbase[j] <= r
     while (hi - lo > 1) {
       const uint32_t m = (lo + hi) >> 1;
@@ -1252,9 +1381,7 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
   kill_empty();
   const double avg_deg = n ? (double)g->nnz / n : 0;
   const uint32_t RG = avg_deg <= 12 ? 8 : (avg_deg <= 40 ? 16 : 64);
-  // the batch caps refinement at two passes: later passes remove little and
-  // cost a full sweep each over every query's candidates
-  const uint32_t bpasses = std::min<uint32_t>(c->tune.refine_passes, 2);
+  const uint32_t bpasses = refine_passes(c, true);
   for (uint32_t pass = 0; any_target && pass < bpasses; pass++) {
     KTimer tr(c, FSIG_K_REFINE);
     std::vector<BClass> cl;
@@ -1340,17 +1467,44 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
         pd.bpos = it->second;
       }
     }
+    // sources: every (query, class a) with its targets, one sweep each
+    std::vector<BSrc> srcs;
+    uint64_t sflat = 0;
+    for (uint32_t i = 0; i < nq; i++) {
+      if (!active[i]) continue;
+      for (uint32_t a = 0; a < plans[i].nclass; a++) {
+        BSrc sc;
+        std::memset(&sc, 0, sizeof(sc));
+        for (uint32_t b = 0; b < plans[i].nclass; b++) {
+          const int pi = pair_of[i][a * MAXQ + b];
+          if (pi < 0) continue;
+          sc.tmask |= 1u << b;
+          sc.pidx[b] = (uint32_t)pi;
+        }
+        if (!sc.tmask) continue;
+        const uint32_t ga = cls_base[i] + a;
+        sc.acand = cand[ga];
+        sc.mask = masks + (uint64_t)i * n;
+        sc.asize = size[ga];
+        sc.flat = sflat;
+        sflat += size[ga];
+        if (size[ga]) srcs.push_back(sc);
+      }
+    }
     const BPair* dP = upload_table(c, pairs);
+    const BSrc* dS = upload_table(c, srcs);
+    const uint32_t nsrc = (uint32_t)srcs.size();
     uint32_t* cnt = (uint32_t*)A.alloc(4ull * std::max<uint64_t>(pflat, 1));
     uint64_t* gscan = (uint64_t*)A.alloc(8ull * (pflat + 1));
     unsigned int* dmax = (unsigned int*)A.alloc(4ull * np);
     uint64_t* pb = (uint64_t*)A.alloc(8ull * np);
-    if (!dP || !cnt || !gscan || !dmax || !pb) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted");
-    k_zero_u32<<<(np + 255) / 256, 256, 0, c->stream>>>(dmax, np);
-    switch (RG) {
-      case 8: k_bcedge_count<8><<<bgrid(c, pflat, FSIG_BLOCK / 8), FSIG_BLOCK, 0, c->stream>>>(dP, np, pflat, g->d_off, g->d_adj, cnt, dmax); break;
-      case 16: k_bcedge_count<16><<<bgrid(c, pflat, FSIG_BLOCK / 16), FSIG_BLOCK, 0, c->stream>>>(dP, np, pflat, g->d_off, g->d_adj, cnt, dmax); break;
-      default: k_bcedge_count<64><<<bgrid(c, pflat, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dP, np, pflat, g->d_off, g->d_adj, cnt, dmax); break;
+    if (!dP || !dS || !cnt || !gscan || !dmax || !pb) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted");
+    if (sflat) {
+      switch (RG) {
+        case 8: k_bcedge_count_multi<8><<<bgrid(c, sflat, FSIG_BLOCK / 8), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, cnt); break;
+        case 16: k_bcedge_count_multi<16><<<bgrid(c, sflat, FSIG_BLOCK / 16), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, cnt); break;
+        default: k_bcedge_count_multi<64><<<bgrid(c, sflat, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, cnt); break;
+      }
     }
     FSIG_HIP(c, hipGetLastError());
     if ((rc = scan_u32_to_u64(c, cnt, gscan, pflat))) return rc;
@@ -1358,11 +1512,14 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
     if ((rc = read_u64(c, gscan + pflat, &total_edges, 1))) return rc;
     lst_all = (uint32_t*)A.alloc(4ull * std::max<uint64_t>(total_edges, 1));
     if (!lst_all) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted for candidate lists");
-    switch (RG) {
-      case 8: k_bcedge_fill<8><<<bgrid(c, pflat, FSIG_BLOCK / 8), FSIG_BLOCK, 0, c->stream>>>(dP, np, pflat, g->d_off, g->d_adj, gscan, lst_all); break;
-      case 16: k_bcedge_fill<16><<<bgrid(c, pflat, FSIG_BLOCK / 16), FSIG_BLOCK, 0, c->stream>>>(dP, np, pflat, g->d_off, g->d_adj, gscan, lst_all); break;
-      default: k_bcedge_fill<64><<<bgrid(c, pflat, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dP, np, pflat, g->d_off, g->d_adj, gscan, lst_all); break;
+    if (sflat) {
+      switch (RG) {
+        case 8: k_bcedge_fill_multi<8><<<bgrid(c, sflat, FSIG_BLOCK / 8), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, gscan, lst_all); break;
+        case 16: k_bcedge_fill_multi<16><<<bgrid(c, sflat, FSIG_BLOCK / 16), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, gscan, lst_all); break;
+        default: k_bcedge_fill_multi<64><<<bgrid(c, sflat, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, gscan, lst_all); break;
+      }
     }
+    k_bpair_maxlen<<<np, FSIG_BLOCK, 0, c->stream>>>(dP, gscan, dmax);
     k_bpair_base<<<(np + 255) / 256, 256, 0, c->stream>>>(gscan, dP, np, pb);
     FSIG_HIP(c, hipGetLastError());
     for (uint32_t i = 0; i < np; i += kPinnedWords)
@@ -1514,13 +1671,32 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
     return M;
   };
   unsigned long long* stats_ls = family_counters(c, FSIG_K_BATCH);
+  // Lockstep batched BFS.  G = 0: lanes by role at each level: 64 for the
+  // queries whose last two levels are counted here (long shared
+  // intersections, a 1024-word LDS tile), 16 for the others.  The live
+  // queries are ordered by size, so each role owns a contiguous row range of
+  // every level (one launch per role).
   auto lockstep = [&](uint32_t G, const std::vector<uint32_t>& qs, float& ms) -> int {
     const size_t mk = A.mark();
     std::vector<uint32_t> live;
     uint32_t maxk = 0;
     for (uint32_t i : qs)
       if (active[i] && ks[i] >= 2) { live.push_back(i); maxk = std::max(maxk, ks[i]); }
+    std::stable_sort(live.begin(), live.end(), [&](uint32_t x, uint32_t y) { return ks[x] < ks[y]; });
     const uint32_t nl = (uint32_t)live.size();
+    auto lanes_of = [&](uint32_t i, uint32_t p) { return G ? G : (ks[i] == p + 2 ? 64u : 16u); };
+    // lane groups of level p: [gstart[j], gstart[j+1]) of live with glanes[j]
+    std::vector<uint32_t> gstart, glanes;
+    auto make_groups = [&](uint32_t p) {
+      gstart.clear();
+      glanes.clear();
+      for (uint32_t j = 0; j < nl; j++)
+        if (j == 0 || lanes_of(live[j], p) != lanes_of(live[j - 1], p)) {
+          gstart.push_back(j);
+          glanes.push_back(lanes_of(live[j], p));
+        }
+      gstart.push_back(nl);
+    };
     FSIG_HIP(c, hipEventRecord(c->ev_a, c->stream));
     if (nl) {
       std::vector<uint64_t> qb(nl + 1, 0);
@@ -1537,29 +1713,47 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
       std::vector<MLevel> ml(nl);
       uint64_t prev_bytes = 0;  // debug timing: device byte counter before the level
       if (clk.on && stats_ls) (void)read_u64(c, stats_ls, &prev_bytes, 1);
+      // one launch per lane-count group over its row range of the level
+      auto launch = [&](bool write, uint32_t p, uint32_t* cnt, uint64_t* off, uint32_t* out) -> int {
+        for (size_t gi = 0; gi + 1 < gstart.size(); gi++) {
+          const uint64_t lo = qb[gstart[gi]], hi = qb[gstart[gi + 1]];
+          if (hi <= lo) continue;
+          const uint32_t GG = glanes[gi];
+          const uint32_t grid = bgrid(c, hi - lo, FSIG_BLOCK / GG);
+#define FSIG_LS(GV, W) k_join_multi<GV, W><<<grid, FSIG_BLOCK, 0, c->stream>>>(dml, dqb, nl, rows, lo, hi, p, cnt, off, out, dcounts, stats_ls)
+          if (!write) {
+            switch (GG) { case 4: FSIG_LS(4, false); break; case 8: FSIG_LS(8, false); break;
+                          case 16: FSIG_LS(16, false); break; case 32: FSIG_LS(32, false); break;
+                          default: FSIG_LS(64, false); break; }
+          } else {
+            switch (GG) { case 4: FSIG_LS(4, true); break; case 8: FSIG_LS(8, true); break;
+                          case 16: FSIG_LS(16, true); break; case 32: FSIG_LS(32, true); break;
+                          default: FSIG_LS(64, true); break; }
+          }
+#undef FSIG_LS
+          FSIG_HIP(c, hipGetLastError());
+        }
+        return FSIG_OK;
+      };
       for (uint32_t p = 1; p < maxk && R > 0; p++) {
         if (clk.on) {
           clk.mark("lockstep level start");
           if (stats_ls) (void)read_u64(c, stats_ls, &prev_bytes, 1);
         }
         bool any_more = false;
+        make_groups(p);
         for (uint32_t j = 0; j < nl; j++) {
-          ml[j] = make_level(live[j], p, G >= 8 && c->tune.fused_tail == 0);
+          ml[j] = make_level(live[j], p, lanes_of(live[j], p) >= 8 && c->tune.fused_tail == 0);
           any_more |= ml[j].last == 0;  // tail rows may defer children to the last level
         }
         FSIG_HIP(c, hipMemcpyAsync(dml, ml.data(), sizeof(MLevel) * nl, hipMemcpyHostToDevice, c->stream));
         uint32_t* cnt = (uint32_t*)A.alloc(4ull * R);
         uint64_t* off = (uint64_t*)A.alloc(8ull * (R + 1));
         if (!cnt || !off) { A.release(mk); return set_err(c, FSIG_ENOMEM, "lockstep: arena exhausted"); }
-        const uint32_t grid = bgrid(c, R, FSIG_BLOCK / G);
-#define FSIG_LS(GG, W) k_join_multi<GG, W><<<grid, FSIG_BLOCK, 0, c->stream>>>(dml, dqb, nl, rows, R, p, cnt, off, out, dcounts, stats_ls)
-        uint32_t* out = nullptr;
-        switch (G) { case 4: FSIG_LS(4, false); break; case 8: FSIG_LS(8, false); break;
-                     case 16: FSIG_LS(16, false); break; default: FSIG_LS(64, false); break; }
-        FSIG_HIP(c, hipGetLastError());
-        if (!any_more) break;
-        int rc2 = scan_u32_to_u64(c, cnt, off, R);
+        int rc2 = launch(false, p, cnt, off, nullptr);
         if (rc2) { A.release(mk); return rc2; }
+        if (!any_more) break;
+        if ((rc2 = scan_u32_to_u64(c, cnt, off, R))) { A.release(mk); return rc2; }
         uint64_t T;
         if ((rc2 = read_u64(c, off + R, &T, 1))) { A.release(mk); return rc2; }
         if (clk.on) {
@@ -1571,14 +1765,18 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
           clk.mark("  count + scan");
         }
         if (T == 0) break;
-        out = (uint32_t*)A.alloc(4ull * (p + 1) * T);
+        uint32_t* out = (uint32_t*)A.alloc(4ull * (p + 1) * T);
         if (!out) { A.release(mk); return set_err(c, FSIG_ENOMEM, "lockstep: a level does not fit the arena"); }
-        switch (G) { case 4: FSIG_LS(4, true); break; case 8: FSIG_LS(8, true); break;
-                     case 16: FSIG_LS(16, true); break; default: FSIG_LS(64, true); break; }
-#undef FSIG_LS
+        if ((rc2 = launch(true, p, cnt, off, out))) { A.release(mk); return rc2; }
         k_gather_bases<<<(nl + 256) / 256, 256, 0, c->stream>>>(off, dqb, nl, dqb2);
         FSIG_HIP(c, hipGetLastError());
         std::swap(dqb, dqb2);
+        // row bases of the next level, for the per-group launch ranges
+        for (uint32_t j = 0; j <= nl; j += kPinnedWords)
+          if ((rc2 = read_u64(c, dqb + j, qb.data() + j, std::min<size_t>(kPinnedWords, nl + 1 - j)))) {
+            A.release(mk);
+            return rc2;
+          }
         rows = out;
         R = T;
       }
@@ -1647,7 +1845,9 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
     if (active[i]) all_q.push_back(i);
   const uint64_t shape = ((uint64_t)nq << 40) ^ ((uint64_t)g->n << 8) ^ (uint64_t)stack_words;
   TuneChoice best{16, false, 0, true};
-  if (c->tune.lanes_per_row) {
+  if (c->tune.strategy == 3) {
+    best = TuneChoice{0, false, 0, true};  // lockstep, lanes per query
+  } else if (c->tune.lanes_per_row) {
     best = TuneChoice{c->tune.lanes_per_row, c->tune.strategy == 2, c->tune.lds_tile ? c->tune.lds_tile : 256,
                       c->tune.strategy != 1 && c->tune.strategy != 2};
   } else {
@@ -1658,20 +1858,37 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
       if (it != g_tuned.end()) { best = it->second; cached = true; }
     }
     if (!cached) {
-      // sample: every 8th live query; both engines, several shapes each
+      // The lockstep shapes are timed on the whole batch (a few heavy
+      // queries can dominate it and a sample may miss them).  The persistent
+      // walk is timed on every 8th query and scaled by the best lockstep
+      // shape's whole-batch / sample ratio.
       std::vector<uint32_t> sample;
       for (size_t j = 0; j < all_q.size(); j += 8) sample.push_back(all_q[j]);
-      const TuneChoice cands[] = {{4, false, 0, true},   {8, false, 0, true},   {16, false, 0, true},
-                                  {64, false, 0, true},  {16, false, 0, false}, {64, false, 0, false},
-                                  {16, true, 256, false}};
+      const TuneChoice ls_cands[] = {{0, false, 0, true}, {8, false, 0, true}, {16, false, 0, true},
+                                     {64, false, 0, true}};
+      const TuneChoice ps_cands[] = {{16, false, 0, false}, {64, false, 0, false}, {16, true, 256, false}};
       float best_ms = 1e30f;
-      for (const TuneChoice& tc : cands) {
+      for (const TuneChoice& tc : ls_cands) {
         float ms = 0;
         FSIG_HIP(c, hipMemsetAsync(dcounts, 0, 8ull * nq, c->stream));
-        rc = tc.lockstep ? lockstep(tc.G, sample, ms) : persistent(tc, sample, false, ms, nullptr);
-        if (rc == FSIG_ENOMEM) continue;  // this engine cannot hold the sample
+        rc = lockstep(tc.G, all_q, ms);
+        if (rc == FSIG_ENOMEM) continue;  // a level does not fit: the walk needs no tables
         if (rc) return rc;
         if (ms < best_ms) { best_ms = ms; best = tc; }
+      }
+      float ls_sample = 0;
+      if (best_ms < 1e30f) {
+        FSIG_HIP(c, hipMemsetAsync(dcounts, 0, 8ull * nq, c->stream));
+        if ((rc = lockstep(best.G, sample, ls_sample)) && rc != FSIG_ENOMEM) return rc;
+      }
+      const double scale = (best_ms < 1e30f && ls_sample > 0) ? (double)best_ms / ls_sample : 8.0;
+      for (const TuneChoice& tc : ps_cands) {
+        float ms = 0;
+        FSIG_HIP(c, hipMemsetAsync(dcounts, 0, 8ull * nq, c->stream));
+        rc = persistent(tc, sample, false, ms, nullptr);
+        if (rc == FSIG_ENOMEM) continue;
+        if (rc) return rc;
+        if (ms * scale < best_ms) { best_ms = (float)(ms * scale); best = tc; }
       }
       std::lock_guard<std::mutex> lk(g_tuned_mu);
       g_tuned[{c, shape}] = best;

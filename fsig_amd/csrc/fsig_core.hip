@@ -457,11 +457,10 @@ int fsig_set_tune(fsig_ctx* c, const fsig_tune* t) {
   if (!c || !t) return FSIG_EINVAL;
   const uint32_t g = t->lanes_per_row;
   if (g != 0 && g != 4 && g != 8 && g != 16 && g != 32 && g != 64) return set_err(c, FSIG_EINVAL, "tune: lanes_per_row");
-  if (t->strategy > 2) return set_err(c, FSIG_EINVAL, "tune: strategy");
+  if (t->strategy > 3) return set_err(c, FSIG_EINVAL, "tune: strategy");
   if (t->fused_tail > 1) return set_err(c, FSIG_EINVAL, "tune: fused_tail");
   if (t->nlf_filter > 1) return set_err(c, FSIG_EINVAL, "tune: nlf_filter");
   c->tune = *t;
-  if (c->tune.refine_passes == 0) c->tune.refine_passes = 4;
   return FSIG_OK;
 }
 

@@ -269,7 +269,7 @@ extern "C" int fsig_server_create(int device, const fsig_graph* g, uint32_t work
   // a batch is admitted against 3/4 of its worker's arena: the rest covers
   // the tables and scan temporaries the projection leaves out
   s->share = per / 4 * 3;
-  fsig_tune t{16, 0, 0, 4, 0, 0};  // lockstep join, 16 lanes per row
+  fsig_tune t{0, 3, 0, 0, 0, 0};  // lockstep join, lanes per query, no autotuning
   if (tune) t = *tune;
   for (uint32_t w = 0; w < workers; w++) {
     fsig_ctx* c = nullptr;

@@ -64,7 +64,7 @@ struct fsig_ctx {
   // resolved after the stream drains (no synchronisation per launch).
   std::vector<TimedLaunch> timed;
   size_t timed_used = 0;
-  fsig_tune tune{0, 0, 0, 4, 0, 0};
+  fsig_tune tune{0, 0, 0, 0, 0, 0};
   std::vector<uint32_t> last_cand;
   uint64_t last_peak = 0;
   // device counters (64 words): [0..7] per-launch results, [16 + 4f ..] per
@@ -183,6 +183,11 @@ struct KTimer {
 // the context statistics; the stream must be idle.
 int resolve_timings(fsig_ctx* c);
 inline unsigned long long* family_counters(fsig_ctx* c, int fam) { return c->d_counters + 16 + 4 * fam; }
+// Refinement passes: the tuned cap, else 4 for one query and 1 for a batch
+// (its neighbour-label filter leaves little for later passes to remove).
+inline uint32_t refine_passes(const fsig_ctx* c, bool batch) {
+  return c->tune.refine_passes ? c->tune.refine_passes : (batch ? 1u : 4u);
+}
 
 // Device-wide exclusive scan of n u32 counts into out[0..n] (u64, out[n] is
 // the total).  Temporary storage comes from the arena.

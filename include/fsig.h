@@ -148,15 +148,18 @@ uint64_t fsig_last_peak_bytes(const fsig_ctx* ctx);
 /* Join tuning, all fields set at once; 0 selects the automatic choice.
  * lanes_per_row in {4,8,16,32,64}: lanes of a wavefront that share one
  * partial embedding (auto: the power of two at or above the mean list
- * length).  strategy (fsig_match_batch only): 0 lockstep join, 1 persistent
- * walk, 2 persistent walk with the second list staged in LDS (lds_tile words
- * per group).  For fsig_match, lds_tile caps the LDS tile of the fused tail
- * (words per lane group).  refine_passes: cap on refinement passes (0 = 4). */
+ * length).  strategy (fsig_match_batch only; with lanes_per_row != 0): 0
+ * lockstep join, 1 persistent walk, 2 persistent walk with the second list
+ * staged in LDS (lds_tile words per group); 3 (any lanes_per_row): lockstep
+ * join with each query's lanes from its mean candidate-list length, no
+ * autotuning.  For fsig_match, lds_tile caps the LDS tile of the fused tail
+ * (words per lane group).  refine_passes: cap on refinement passes (0: 4
+ * for fsig_match, 1 for a batch). */
 typedef struct fsig_tune {
   uint32_t lanes_per_row;
   uint32_t strategy;
   uint32_t lds_tile;
-  uint32_t refine_passes; /* cap on refinement passes (default 4) */
+  uint32_t refine_passes; /* cap on refinement passes (0: default) */
   uint32_t fused_tail;    /* 0: count the last two levels of a count-mode
                              alg-1 match in one pass (the next-to-last level
                              is never written) when the last query vertex has
@@ -194,7 +197,7 @@ int fsig_stream_copy(fsig_ctx* ctx, uint64_t bytes, uint32_t iters, double* gbps
  * wait only for the next free worker, and the batches of different workers
  * overlap on the device.  A batch that still runs out of arena falls back
  * to fsig_match per query, whose join is memory-blocked.  tune (NULL:
- * lockstep join, 16 lanes per row, no autotuning) is set on every worker.
+ * lockstep join with lanes per query, no autotuning) is set on every worker.
  * Replaces the reference's one query per process with a static device
  * budget (main/run.cc:101-140, memory_manager/memory_manager.h:220-242). */
 typedef struct fsig_server_stats {

@@ -74,5 +74,19 @@ def test_batch_autotuner_and_repeat(gpu_ctx):
     b, _ = F.match_batch(dg, qs)   # second call reuses the tuning decision
     assert a.tolist() == b.tolist()
     lanes, tile, engine = F.last_batch_tune(gpu_ctx)
-    assert lanes in (4, 8, 16, 32, 64) and engine in ("lockstep", "persistent")
+    # lanes 0: the lockstep join with each query's own lane count
+    assert lanes in (0, 4, 8, 16, 32, 64) and engine in ("lockstep", "persistent")
     assert a.tolist() == list(ref)
+
+
+def test_batch_lanes_per_query(gpu_ctx):
+    g = F.HostGraph.chung_lu(30000, 200000, 2.3, 15).set_labels("zipf", 16, 1.2, 16)
+    dg = gpu_ctx.upload(g)
+    qs, ref = zip(*_bounded(dg, [q for _, q in W.walk_queries(g, 48, 3, 12, 17)]))
+    try:
+        gpu_ctx.set_tune(strategy=3)
+        counts, _ = F.match_batch(dg, qs)
+        assert F.last_batch_tune(gpu_ctx)[0] == 0
+    finally:
+        gpu_ctx.set_tune()
+    assert counts.tolist() == list(ref)

@@ -97,7 +97,7 @@ def test_candidate_sets_equal_oracle_refinement(gpu_ctx):
     g = F.HostGraph.chung_lu(8000, 40000, 2.5, 31).set_labels("uniform", 4, seed=32)
     off, adj, lab = g.arrays()
     dg = gpu_ctx.upload(g)
-    passes = gpu_ctx.get_tune()["refine_passes"]
+    passes = gpu_ctx.get_tune()["refine_passes"] or 4
     gpu_ctx.set_tune(nlf_filter=1)  # the reference's filters only
     try:
         for s in range(5):
