@@ -926,13 +926,15 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join_multi(const MLevel* __restr
                                                            const uint64_t* __restrict__ off,
                                                            uint32_t* __restrict__ out,
                                                            unsigned long long* __restrict__ qcounts,
-                                                           unsigned long long* __restrict__ stats) {
+                                                           unsigned long long* __restrict__ stats,
+                                                           uint32_t tile_words) {
   constexpr int GPB = FSIG_BLOCK / G;
   constexpr int NR = (MAXQ + G - 1) / G;
   // the count pass of G >= 8 also runs fused tails (their LDS tiles)
   constexpr bool TAIL = !WRITE && G >= 8;
   using Sh = typename std::conditional<TAIL, TailShared<G>, PlainShared<G>>::type;
   __shared__ Sh shm;
+  extern __shared__ uint32_t tail_tile[];
   auto& sh_s = shm.s1;
   auto& sh_n = shm.n1;
   auto& sh_cf = shm.cf1;
@@ -964,7 +966,7 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join_multi(const MLevel* __restr
       if (M.tail) {
         uint64_t kids = 0;
         bool defer = false;
-        const uint64_t leaves = group_sum_u64<G>(tail_row<G>(L, M.S, M.L2, M.S2, rv, shm.at(gw), gl, gmask, below,
+        const uint64_t leaves = group_sum_u64<G>(tail_row<G>(L, M.S, M.L2, M.S2, rv, shm.at(gw, tail_tile, tile_words), gl, gmask, below,
                                                              kids, defer, my_bytes));
         if (gl == 0) {
           my_units += 1;
@@ -1720,7 +1722,13 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
           if (hi <= lo) continue;
           const uint32_t GG = glanes[gi];
           const uint32_t grid = bgrid(c, hi - lo, FSIG_BLOCK / GG);
-#define FSIG_LS(GV, W) k_join_multi<GV, W><<<grid, FSIG_BLOCK, 0, c->stream>>>(dml, dqb, nl, rows, lo, hi, p, cnt, off, out, dcounts, stats_ls)
+          // tile of the fused tails among this group's queries
+          uint32_t tw = 0;
+          if (!write && GG >= 8)
+            for (uint32_t j = gstart[gi]; j < gstart[gi + 1]; j++)
+              if (ml[j].tail) tw = std::max(tw, tail_tile_words(GG, ml[j].L2, 0));
+          const size_t lds = (size_t)tw * (FSIG_BLOCK / GG) * sizeof(uint32_t);
+#define FSIG_LS(GV, W) k_join_multi<GV, W><<<grid, FSIG_BLOCK, lds, c->stream>>>(dml, dqb, nl, rows, lo, hi, p, cnt, off, out, dcounts, stats_ls, tw)
           if (!write) {
             switch (GG) { case 4: FSIG_LS(4, false); break; case 8: FSIG_LS(8, false); break;
                           case 16: FSIG_LS(16, false); break; case 32: FSIG_LS(32, false); break;

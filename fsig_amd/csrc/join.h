@@ -507,16 +507,21 @@ __device__ __forceinline__ uint64_t tail_row(const JoinLevel& L1, const CandEdge
   return leaves;
 }
 
+// Per-group LDS of the fused tail: list ranges and conflicts of both
+// levels (at most kTailMax entries each) and the shared-intersection tile,
+// which lives in dynamic LDS sized per launch (0 when no row of the launch
+// needs it).
+constexpr uint32_t kTailMax = 16;
+
 template <int G>
 struct TailShared {
   static constexpr int GPB = FSIG_BLOCK / G;
-  static constexpr uint32_t kTileWords = 4096 / GPB;  // 16 KiB of tiles per block
-  uint32_t s1[GPB][MAXQ], n1[GPB][MAXQ], cf1[GPB][MAXQ];
-  uint32_t s2[GPB][MAXQ], n2[GPB][MAXQ], cf2[GPB][MAXQ];
+  static constexpr uint32_t kTileWords = 4096 / GPB;  // at most 16 KiB of tiles per block
+  uint32_t s1[GPB][MAXQ], n1[GPB][MAXQ], cf1[GPB][MAXQ];  // also the plain rows of a lockstep launch
+  uint32_t s2[GPB][kTailMax], n2[GPB][kTailMax], cf2[GPB][kTailMax];
   uint32_t slot[GPB][G];
-  uint32_t tile[GPB][kTileWords];
-  __device__ TailLds at(uint32_t gw) {
-    return TailLds{s1[gw], n1[gw], cf1[gw], s2[gw], n2[gw], cf2[gw], slot[gw], tile[gw], kTileWords};
+  __device__ TailLds at(uint32_t gw, uint32_t* tile, uint32_t tw) {
+    return TailLds{s1[gw], n1[gw], cf1[gw], s2[gw], n2[gw], cf2[gw], slot[gw], tile + gw * tw, tw};
   }
 };
 
@@ -526,7 +531,7 @@ struct TailIO {
   uint32_t* cnt;              // per row: children of a deferred row, else 0
   unsigned long long* res;    // [0] += leaves, [1] += children of counted rows
   unsigned long long* stats;  // [0] += algorithmic bytes, [1] += parent rows
-  uint32_t tile_cap;          // words of the shared-intersection tile (0: all)
+  uint32_t tile_words;        // shared-intersection tile per group (dynamic LDS)
 };
 
 template <int G>
@@ -535,12 +540,12 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join_tail(JoinLevel L1, CandEdge
   constexpr int GPB = FSIG_BLOCK / G;
   constexpr int NR = (MAXQ + G - 1) / G;
   __shared__ TailShared<G> sh;
+  extern __shared__ uint32_t tail_tile[];
   const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
   const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
   const uint64_t below = (1ull << lane) - 1;
   const uint64_t ngroups = (uint64_t)gridDim.x * GPB;
-  TailLds t = sh.at(gw);
-  if (io.tile_cap && io.tile_cap < t.tw) t.tw = io.tile_cap;
+  const TailLds t = sh.at(gw, tail_tile, io.tile_words);
   unsigned long long my_leaves = 0, my_kids = 0, my_bytes = 0, my_rows = 0;
   for (uint64_t r = (uint64_t)xcd_block() * GPB + gw; r < io.nrows; r += ngroups) {
     uint32_t rv[NR];
@@ -573,8 +578,19 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join_tail(JoinLevel L1, CandEdge
   }
 }
 
-// Whether the fused tail applies to leaf level L2 (it needs a backward list).
-inline bool tail_applies(const JoinLevel& L2) { return L2.nb >= 1; }
+// Whether the fused tail applies to leaf level L2: it needs a backward list,
+// and its lists and conflicts must fit the group arrays.
+inline bool tail_applies(const JoinLevel& L2) { return L2.nb >= 1 && L2.nb <= kTailMax && L2.ncf <= kTailMax; }
+
+// Tile words per group the tail of L2 needs: only a leaf with both shared
+// lists (parent columns) and an own list (column p) intersects in LDS.
+inline uint32_t tail_tile_words(uint32_t G, const JoinLevel& L2, uint32_t cap) {
+  bool shared = false, own = false;
+  for (uint32_t i = 0; i < L2.nb; i++) (L2.bcol[i] == L2.p - 1 ? own : shared) = true;
+  if (!shared || !own) return 0;
+  const uint32_t w = 4096 / (FSIG_BLOCK / G);
+  return cap && cap < w ? cap : w;
+}
 
 inline int launch_join_tail(fsig_ctx* c, uint32_t G, const JoinLevel& L1, const CandEdgeSrc& S1,
                             const JoinLevel& L2, const CandEdgeSrc& S2, const TailIO& io) {
@@ -583,12 +599,13 @@ inline int launch_join_tail(fsig_ctx* c, uint32_t G, const JoinLevel& L1, const 
   uint64_t blocks = std::min<uint64_t>((io.nrows + gpb - 1) / gpb, (uint64_t)c->num_cus * 16);
   if (blocks >= 64) blocks &= ~7ull;
   const uint32_t grid = (uint32_t)std::max<uint64_t>(blocks, 1);
+  const size_t lds = (size_t)io.tile_words * gpb * sizeof(uint32_t);
   switch (G) {
-    case 4: k_join_tail<4><<<grid, FSIG_BLOCK, 0, c->stream>>>(L1, S1, L2, S2, io); break;
-    case 8: k_join_tail<8><<<grid, FSIG_BLOCK, 0, c->stream>>>(L1, S1, L2, S2, io); break;
-    case 16: k_join_tail<16><<<grid, FSIG_BLOCK, 0, c->stream>>>(L1, S1, L2, S2, io); break;
-    case 32: k_join_tail<32><<<grid, FSIG_BLOCK, 0, c->stream>>>(L1, S1, L2, S2, io); break;
-    default: k_join_tail<64><<<grid, FSIG_BLOCK, 0, c->stream>>>(L1, S1, L2, S2, io); break;
+    case 4: k_join_tail<4><<<grid, FSIG_BLOCK, lds, c->stream>>>(L1, S1, L2, S2, io); break;
+    case 8: k_join_tail<8><<<grid, FSIG_BLOCK, lds, c->stream>>>(L1, S1, L2, S2, io); break;
+    case 16: k_join_tail<16><<<grid, FSIG_BLOCK, lds, c->stream>>>(L1, S1, L2, S2, io); break;
+    case 32: k_join_tail<32><<<grid, FSIG_BLOCK, lds, c->stream>>>(L1, S1, L2, S2, io); break;
+    default: k_join_tail<64><<<grid, FSIG_BLOCK, lds, c->stream>>>(L1, S1, L2, S2, io); break;
   }
   FSIG_HIP(c, hipGetLastError());
   return FSIG_OK;
@@ -757,9 +774,11 @@ class JoinDriver {
                  const JoinLevel& L2, const CandEdgeSrc& S2) {
     unsigned long long* res = c_->d_counters;
     FSIG_HIP(c_, hipMemsetAsync(res, 0, 2 * sizeof(unsigned long long), c_->stream));
-    const TailIO io{rows, R, cnt, res, family_counters(c_, FSIG_K_JOIN_COUNT), c_->tune.lds_tile};
+    const uint32_t G = std::max<uint32_t>(G_, 8);
+    const TailIO io{rows, R, cnt, res, family_counters(c_, FSIG_K_JOIN_COUNT),
+                    tail_tile_words(G, L2, c_->tune.lds_tile)};
     KTimer t(c_, FSIG_K_JOIN_COUNT);
-    int rc = launch_join_tail(c_, std::max<uint32_t>(G_, 8), L1, S1, L2, S2, io);
+    int rc = launch_join_tail(c_, G, L1, S1, L2, S2, io);
     if (rc) return rc;
     t.done(0, 0);
     uint64_t h[2];
