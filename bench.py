@@ -27,6 +27,23 @@ sys.path.insert(0, ROOT)
 
 METRIC = "embeddings/s & queries/s (batched vs per-query), p50/p99 latency, % HBM peak"
 HBM_PEAK_GBS = 8000.0
+KERNEL_OF_FAMILY = {"join_tail": "k_join_tail (fused last two levels)", "join_count": "k_join (count pass)",
+                    "join_write": "k_join (write pass)", "refine": "k_refine", "cedge": "k_cedge_count/fill",
+                    "filter": "k_ldf", "batch": "k_join_multi"}
+COUNTERS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "c2_hbm_counters.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy-kernel
+    calibration), or None when the summary does not cover it."""
+    try:
+        with open(COUNTERS) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = rec.get("kernels", {}).get(kernel)
+    return None if k is None else int(k["fetch_bytes"] + k["write_bytes"])
 
 
 def log(*a):
@@ -150,8 +167,10 @@ def main():
     dom = max((f for f in stats if stats[f]["launches"]), key=lambda f: stats[f]["ms"])
     d = stats[dom]
     achieved = d["alg_bytes"] / (d["ms"] / 1e3) / 1e9
-    roofline = {"bound": "hbm", "kernel": f"k_join ({dom} pass)", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+    kname = KERNEL_OF_FAMILY.get(dom, dom)
+    roofline = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(kname),
                 "avg_launch_ms": round(d["ms"] / d["launches"], 4),
                 "alg_bytes_per_launch": int(d["alg_bytes"] / d["launches"])}
 

@@ -30,8 +30,8 @@ FSIG_OK, FSIG_EINVAL, FSIG_ENOMEM, FSIG_EHIP, FSIG_EUNSUPPORTED, FSIG_ENOTFOUND,
 FSIG_ELIMIT = -7
 ALG_MMATCH, ALG_GSI = 1, 2
 MODE_COUNT, MODE_ENUMERATE = 0, 1
-K_FILTER, K_REFINE, K_CEDGE, K_JOIN_COUNT, K_JOIN_WRITE, K_SCAN, K_BATCH = range(7)
-KERNEL_FAMILIES = ["filter", "refine", "cedge", "join_count", "join_write", "scan", "batch"]
+K_FILTER, K_REFINE, K_CEDGE, K_JOIN_COUNT, K_JOIN_WRITE, K_SCAN, K_BATCH, K_JOIN_TAIL = range(8)
+KERNEL_FAMILIES = ["filter", "refine", "cedge", "join_count", "join_write", "scan", "batch", "join_tail"]
 
 _u32p = C.POINTER(C.c_uint32)
 _i32p = C.POINTER(C.c_int32)

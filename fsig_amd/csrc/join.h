@@ -775,9 +775,9 @@ class JoinDriver {
     unsigned long long* res = c_->d_counters;
     FSIG_HIP(c_, hipMemsetAsync(res, 0, 2 * sizeof(unsigned long long), c_->stream));
     const uint32_t G = std::max<uint32_t>(G_, 8);
-    const TailIO io{rows, R, cnt, res, family_counters(c_, FSIG_K_JOIN_COUNT),
+    const TailIO io{rows, R, cnt, res, family_counters(c_, FSIG_K_JOIN_TAIL),
                     tail_tile_words(G, L2, c_->tune.lds_tile)};
-    KTimer t(c_, FSIG_K_JOIN_COUNT);
+    KTimer t(c_, FSIG_K_JOIN_TAIL);
     int rc = launch_join_tail(c_, G, L1, S1, L2, S2, io);
     if (rc) return rc;
     t.done(0, 0);

@@ -130,8 +130,9 @@ enum {
   FSIG_K_JOIN_COUNT = 3,
   FSIG_K_JOIN_WRITE = 4,
   FSIG_K_SCAN = 5,
-  FSIG_K_BATCH = 6,    /* persistent batched join                        */
-  FSIG_K_NUM = 7
+  FSIG_K_BATCH = 6,    /* batched join (lockstep or persistent)          */
+  FSIG_K_JOIN_TAIL = 7,/* fused count of the last two levels            */
+  FSIG_K_NUM = 8
 };
 int fsig_set_profiling(fsig_ctx* ctx, int on);
 int fsig_stats_reset(fsig_ctx* ctx);
