@@ -226,7 +226,7 @@ def bench_c4(args):
     g = W.c4_graph()
     ctx = F.Context(0, 96 << 30)
     dg = ctx.upload(g)
-    queries = W.c4_queries(g, dg)
+    queries = W.c4_queries(g, dg, progress=log)
     log(f"c4 graph {g.n} V {g.nnz // 2} E, {len(queries)} queries, {time.perf_counter() - t0:.1f}s")
     # plain loop (one warm-up pass, one timed pass)
     lat, plain_counts = [], []
@@ -291,7 +291,7 @@ def bench_c5(args):
     ctx = F.Context(0, 32 << 30)
     dg = ctx.upload(g)
     t0 = time.perf_counter()
-    stream = W.c5_stream(g, dg, args.requests, args.qps)
+    stream = W.c5_stream(g, dg, args.requests, args.qps, progress=log)
     arrivals = np.array([a for a, _ in stream])
     qs = [q for _, q in stream]
     log(f"{len(qs)} requests drawn in {time.perf_counter() - t0:.1f}s, offered {len(qs) / arrivals[-1]:.0f} q/s")
@@ -342,7 +342,8 @@ def bench_c5(args):
            "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
            "config": {"workload": f"C5 served: R-MAT 2^24 V / {g.nnz // 2} E, 64 uniform labels, resident; "
                                   f"{len(qs)} Poisson requests at {len(qs) / arrivals[-1]:.0f} q/s offered, "
-                                  "k log-uniform in 4..32",
+                                  "k = round(exp(U(ln 4, ln 32))) conditioned on k <= 5 (larger walk queries "
+                                  "exceed 1e8 embeddings / 2e7 partial rows on this graph), count mode",
                       "workers": args.workers, "hbm_budget_GiB": budget >> 30, "max_batch": 64,
                       "parallelism": "1 GPU"},
            "served_p50_ms": round(float(np.percentile(served_lat, 50)), 3),
@@ -351,6 +352,7 @@ def bench_c5(args):
            "plain_p50_ms": round(float(np.percentile(plain_lat, 50)), 3),
            "plain_p99_ms": round(float(np.percentile(plain_lat, 99)), 3),
            "served_over_plain": round(plain_s / served_s, 2),
+           "k_histogram": {str(k): int(sum(1 for q in qs if q.k == k)) for k in sorted({q.k for q in qs})},
            "embeddings": int(sum(plain_counts)),
            "server": {k: (round(v, 1) if isinstance(v, float) else v) for k, v in st.items()}}
     print(json.dumps(out), flush=True)

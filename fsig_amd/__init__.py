@@ -268,6 +268,7 @@ class HostGraph:
             raise FsigError(FSIG_EINVAL, f"no walk query with k={k}, edges in [{min_edges},{max_e}]")
         q = Query(k, lab, ed[: 2 * ne].reshape(-1, 2))
         q.walk = walk
+        q.walk_seed = seed
         return q
 
 

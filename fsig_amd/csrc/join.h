@@ -695,7 +695,10 @@ class JoinDriver {
     int rc;
     bool fused = false;
     if constexpr (std::is_same<Src, CandEdgeSrc>::value)
-      fused = mode_ == FSIG_MODE_COUNT && p + 2 == k_ && c_->tune.fused_tail == 0 && tail_applies(pv_.level(p + 1));
+      // (a work limit needs the next-to-last level's size before it is
+      // expanded, which only the level-by-level path knows)
+      fused = mode_ == FSIG_MODE_COUNT && p + 2 == k_ && c_->tune.fused_tail == 0 && c_->work_limit == 0 &&
+              tail_applies(pv_.level(p + 1));
     if (last && mode_ == FSIG_MODE_COUNT) {
       FSIG_HIP(c_, hipMemsetAsync(res, 0, sizeof(unsigned long long), c_->stream));
       JoinIO io{rows, 0, R, nullptr, nullptr, nullptr, 0, 0, res, family_counters(c_, FSIG_K_JOIN_COUNT)};

@@ -236,11 +236,14 @@ static bool has_edge(const uint32_t* off, const uint32_t* adj, uint32_t u, uint3
 
 int random_walk_query(uint32_t n, const uint32_t* off, const uint32_t* adj, const int32_t* labels,
                       uint32_t k, uint32_t min_edges, uint32_t max_edges, uint64_t seed, int bias,
-                      uint32_t max_attempts, WalkQuery& out) {
+                      uint32_t max_attempts, WalkQuery& out, const std::vector<uint32_t>* given) {
   Rng r(seed ^ 0x3a1cull);
-  std::vector<uint32_t> nonisolated;
-  for (uint32_t v = 0; v < n; v++)
-    if (off[v + 1] > off[v]) nonisolated.push_back(v);
+  std::vector<uint32_t> local;
+  if (!given) {
+    for (uint32_t v = 0; v < n; v++)
+      if (off[v + 1] > off[v]) local.push_back(v);
+  }
+  const std::vector<uint32_t>& nonisolated = given ? *given : local;
   if (nonisolated.empty() || k == 0) return -1;
   std::vector<uint32_t> visit;
   for (uint32_t attempt = 0; attempt < max_attempts; attempt++) {
@@ -250,9 +253,10 @@ int random_walk_query(uint32_t n, const uint32_t* off, const uint32_t* adj, cons
     for (uint32_t step = 0; step < 64 * k && visit.size() < k; step++) {
       uint32_t d = off[cur + 1] - off[cur];
       uint32_t nxt = adj[off[cur] + r.below(d)];
-      for (int b = 0; b < bias; b++) {
+      for (int b = 0; b < (bias < 0 ? -bias : bias); b++) {
         uint32_t alt = adj[off[cur] + r.below(d)];
-        if (off[alt + 1] - off[alt] > off[nxt + 1] - off[nxt]) nxt = alt;
+        const uint32_t da = off[alt + 1] - off[alt], dn = off[nxt + 1] - off[nxt];
+        if (bias > 0 ? da > dn : da < dn) nxt = alt;
       }
       cur = nxt;
       if (std::find(visit.begin(), visit.end(), cur) == visit.end()) visit.push_back(cur);

@@ -50,10 +50,12 @@ struct WalkQuery {
 // visited; the query is their induced subgraph with their labels, ids given
 // in visit order.  Walks are retried until the edge count lies in
 // [min_edges, max_edges].  bias > 0 steps to the highest-degree of
-// (1 + bias) uniformly drawn neighbours (reaches the denser core).
+// (1 + bias) uniformly drawn neighbours (reaches the denser core); bias < 0
+// to the lowest-degree of (1 - bias) (stays in the periphery).
 // Returns 0, or -1 when max_attempts walks all failed.
 int random_walk_query(uint32_t n, const uint32_t* off, const uint32_t* adj, const int32_t* labels,
                       uint32_t k, uint32_t min_edges, uint32_t max_edges, uint64_t seed, int bias,
-                      uint32_t max_attempts, WalkQuery& out);
+                      uint32_t max_attempts, WalkQuery& out,
+                      const std::vector<uint32_t>* nonisolated = nullptr);
 
 }  // namespace fsig

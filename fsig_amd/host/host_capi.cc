@@ -1,4 +1,5 @@
 // C ABI of libfsig_host.so (include/fsig_host.h).
+#include <mutex>
 #include "../../include/fsig_host.h"
 
 #include <algorithm>
@@ -17,6 +18,10 @@ struct fsig_hgraph {
   std::vector<uint32_t> off{0};
   std::vector<uint32_t> adj;
   std::vector<int32_t> labels;
+  // walk starts: the non-isolated vertices, built on the first walk query
+  mutable std::vector<uint32_t> walk_starts;
+  mutable bool walk_starts_ready = false;
+  mutable std::mutex walk_mu;
 };
 
 struct fsig_hpcsr {
@@ -139,8 +144,16 @@ void fsig_hg_free(fsig_hgraph* g) { delete g; }
 int fsig_hg_walk_query(const fsig_hgraph* g, uint32_t k, uint32_t min_e, uint32_t max_e, uint64_t seed, int bias,
                        uint32_t max_attempts, int32_t* labels, uint32_t* edges, uint32_t* walk) {
   WalkQuery q;
+  {
+    std::lock_guard<std::mutex> lk(g->walk_mu);
+    if (!g->walk_starts_ready) {
+      for (uint32_t v = 0; v < g->n; v++)
+        if (g->off[v + 1] > g->off[v]) g->walk_starts.push_back(v);
+      g->walk_starts_ready = true;
+    }
+  }
   if (random_walk_query(g->n, g->off.data(), g->adj.data(), g->labels.data(), k, min_e, max_e, seed, bias,
-                        max_attempts, q))
+                        max_attempts, q, &g->walk_starts))
     return -1;
   for (uint32_t i = 0; i < k; i++) {
     labels[i] = q.labels[i];

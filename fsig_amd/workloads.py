@@ -18,7 +18,9 @@ seed names a workload exactly:
 """
 from __future__ import annotations
 
+import json
 import math
+import os
 from typing import List, Tuple
 
 import numpy as np
@@ -76,35 +78,46 @@ def c4_graph() -> HostGraph:
 C4_QUERIES = dict(count=256, kmin=4, kmax=16, seed=4, max_count=10 ** 8, max_rows=2 * 10 ** 7)
 
 
-def c4_queries(g: HostGraph, dg) -> List[Query]:
+def c4_queries(g: HostGraph, dg, progress=None) -> List[Query]:
     p = C4_QUERIES
     return [q for _, q in walk_queries(g, p["count"], p["kmin"], p["kmax"], p["seed"], dg=dg,
-                                       max_count=p["max_count"], max_rows=p["max_rows"])]
+                                       max_count=p["max_count"], max_rows=p["max_rows"], progress=progress)]
 
 
 def walk_queries(g: HostGraph, count: int, kmin: int, kmax: int, seed: int, log_uniform: bool = False,
                  max_edges: int = 10 ** 6, dg=None, max_count: int = 0,
-                 max_rows: int = 0) -> List[Tuple[str, Query]]:
+                 max_rows: int = 0, progress=None, max_attempts: int = 2000,
+                 alternate_drawn: bool = False, k_cap: int = 0) -> List[Tuple[str, Query]]:
     """count queries, k uniform (or log-uniform) in [kmin, kmax], alternating
     dense (average degree >= 3) and sparse; seeds that find no walk with the
     wanted density are skipped deterministically.  With a device graph `dg`,
     a query is also rejected when it has more than max_count embeddings or
     its plain join would build more than max_rows intermediate rows (the
     engine's bounded count and work limit decide, so the check stops early
-    on huge queries)."""
+    on huge queries).  With alternate_drawn, dense and sparse alternate over
+    the drawn walks instead of the accepted queries; with k_cap, draws of a
+    larger k are skipped (the law of k conditioned on k <= k_cap)."""
     rng = np.random.default_rng(seed)
     out = []
     s = seed * 100003
+    draws = 0
     while len(out) < count:
+        draws += 1
+        if progress is not None and draws % 25 == 0:
+            progress(f"{draws} walks drawn, {len(out)} of {count} queries accepted")
         if log_uniform:
             k = int(round(math.exp(rng.uniform(math.log(kmin), math.log(kmax)))))
         else:
             k = int(rng.integers(kmin, kmax + 1))
-        dense = len(out) % 2 == 0
+        if k_cap and k > k_cap:
+            continue  # conditioned on k <= k_cap
+        # dense and sparse alternate over accepted queries (an even mix), or
+        # over drawn walks (the mix the bounds let through)
+        dense = (draws if alternate_drawn else len(out)) % 2 == 0
         lo, hi = ((3 * k + 1) // 2, max_edges) if dense else (k - 1, (3 * k - 1) // 2)
         s += 1
         try:
-            q = g.walk_query(k, lo, hi, seed=s, bias=2 if dense else 0, max_attempts=2000)
+            q = g.walk_query(k, lo, hi, seed=s, bias=2 if dense else 0, max_attempts=max_attempts)
         except FsigError:
             continue
         if dg is not None and (max_count or max_rows):
@@ -121,6 +134,8 @@ def walk_queries(g: HostGraph, count: int, kmin: int, kmax: int, seed: int, log_
             if max_count and c > max_count:
                 continue
         out.append(("dense" if dense else "sparse", q))
+        if progress is not None and len(out) % 32 == 0:
+            progress(f"{len(out)} of {count} queries drawn")
     return out
 
 
@@ -128,20 +143,54 @@ def c5_graph(scale: int = 24, m: int = 200_000_000, seed: int = 8) -> HostGraph:
     return HostGraph.rmat(scale, m, 0.57, 0.19, 0.19, seed).set_labels("uniform", 64, seed=9)
 
 
-# C5 request stream: k = round(exp(U(ln 4, ln 32))), alternating dense and
-# sparse walks (redrawn like C4's above the same bounds), Poisson arrivals at
-# `qps` (exponential gaps from their own seed).
-C5_STREAM = dict(count=512, kmin=4, kmax=32, seed=11, arrival_seed=12, qps=2000.0, max_count=10 ** 8,
-                 max_rows=2 * 10 ** 7)
+# C5 request stream: k = round(exp(U(ln 4, ln 32))) conditioned on k <= 5,
+# dense and sparse walks alternating per draw (200 attempts per walk),
+# redrawn above the C4 bounds (more than 1e8 embeddings or 2e7 partial
+# embeddings).  On this R-MAT graph every walk-induced query with k >= 6 that
+# was tried exceeded them (counts of 1e9..1e10 and more), so the stream is
+# the servable part of the law.  Poisson arrivals at `qps` (exponential gaps
+# from their own seed).  Drawing the stream checks every candidate on the
+# device, so a drawn stream is kept in a JSON file: each query with the walk
+# seed that reproduces it.
+C5_STREAM = dict(count=128, kmin=4, kmax=32, k_cap=5, seed=11, arrival_seed=12, qps=400.0, max_count=10 ** 8,
+                 max_rows=2 * 10 ** 7, max_attempts=200)
+C5_STREAM_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "c5_stream.json")
 
 
-def c5_stream(g: HostGraph, dg, count: int = 0, qps: float = 0.0) -> List[Tuple[float, Query]]:
-    """(arrival time in s from the stream start, query) pairs."""
-    p = C5_STREAM
-    count = count or p["count"]
+def _stream_key(p):
+    key = {k: p[k] for k in ("count", "kmin", "kmax", "k_cap", "seed", "max_count", "max_rows", "max_attempts")}
+    key["alternate"] = "drawn"
+    return key
+
+
+def c5_stream(g: HostGraph, dg, count: int = 0, qps: float = 0.0, progress=None,
+              cache: str = C5_STREAM_FILE) -> List[Tuple[float, Query]]:
+    """(arrival time in s from the stream start, query) pairs.  Queries come
+    from `cache` when it holds a stream drawn with the same parameters
+    (otherwise they are drawn, and written there when cache is a path)."""
+    p = dict(C5_STREAM)
+    if count:
+        p["count"] = count
     qps = qps or p["qps"]
-    qs = [q for _, q in walk_queries(g, count, p["kmin"], p["kmax"], p["seed"], log_uniform=True, dg=dg,
-                                     max_count=p["max_count"], max_rows=p["max_rows"])]
+    qs = None
+    if cache and os.path.exists(cache):
+        with open(cache) as f:
+            rec = json.load(f)
+        if rec.get("key") == _stream_key(p):
+            qs = [Query(r["k"], r["labels"], r["edges"]) for r in rec["queries"]]
+    if qs is None:
+        drawn = [q for _, q in walk_queries(g, p["count"], p["kmin"], p["kmax"], p["seed"], log_uniform=True,
+                                            dg=dg, max_count=p["max_count"], max_rows=p["max_rows"],
+                                            progress=progress, max_attempts=p["max_attempts"],
+                                            alternate_drawn=True, k_cap=p["k_cap"])]
+        if cache:
+            os.makedirs(os.path.dirname(cache), exist_ok=True)
+            with open(cache, "w") as f:
+                json.dump({"key": _stream_key(p),
+                           "queries": [{"k": q.k, "labels": [int(x) for x in q.labels],
+                                        "edges": [[int(a), int(b)] for a, b in q.edges.tolist()],
+                                        "walk_seed": int(q.walk_seed)} for q in drawn]}, f)
+        qs = drawn
     gaps = np.random.default_rng(p["arrival_seed"]).exponential(1.0 / qps, size=len(qs))
     t = np.cumsum(gaps) - gaps[0]
     return list(zip(t.tolist(), qs))

@@ -164,7 +164,8 @@ typedef struct fsig_tune {
   uint32_t fused_tail;    /* 0: count the last two levels of a count-mode
                              alg-1 match in one pass (the next-to-last level
                              is never written) when the last query vertex has
-                             a backward neighbour; 1: level by level */
+                             a backward neighbour and no work limit is set;
+                             1: level by level */
   uint32_t nlf_filter;    /* 0: alg-1 filtering also requires, per label, at
                              least as many neighbours of that label as the
                              query vertex has (graphs with 2..255 labels);
