@@ -676,7 +676,21 @@ class JoinDriver {
   bool bounded() const { return max_emb_ != 0; }
   uint64_t remaining() const { return bounded() ? max_emb_ - total_ : ~0ull; }
 
+  // A bounded match expands its parent rows in chunks that double from 64K
+  // rows, and stops as soon as the bound is reached (an unbounded one takes
+  // every row at once).
   int expand(uint32_t p, const uint32_t* rows, uint64_t R) {
+    if (R == 0) return FSIG_OK;
+    if (p == k_ || !bounded()) return expand_rows(p, rows, R);
+    uint64_t chunk = 1ull << 16;
+    for (uint64_t r0 = 0; r0 < R && remaining() > 0; r0 += chunk, chunk *= 2) {
+      const int rc = expand_rows(p, rows + r0 * p, std::min(chunk, R - r0));
+      if (rc) return rc;
+    }
+    return FSIG_OK;
+  }
+
+  int expand_rows(uint32_t p, const uint32_t* rows, uint64_t R) {
     if (R == 0) return FSIG_OK;
     if (p == k_) {
       uint64_t n = std::min<uint64_t>(R, remaining());
