@@ -417,7 +417,7 @@ __device__ __forceinline__ uint64_t tail_row(const JoinLevel& L1, const CandEdge
     return leaves;
   }
   // A row whose shared intersection does not fit the tile, or that may have
-  // many children while I is large, is deferred: its children are counted here and written by the caller,
+  // very many children (> 16G) while I is large (> 4G), is deferred: its children are counted here and written by the caller,
   // which counts their leaves one row per group.  Other rows stage I in LDS once and count each
   // child's leaves cooperatively: lanes split the shorter of I and the
   // child's own list and search the other.
@@ -442,7 +442,7 @@ __device__ __forceinline__ uint64_t tail_row(const JoinLevel& L1, const CandEdge
     }
     lds_wave_sync();
     // many children, each with a large I to intersect: spread them instead
-    defer_row = defer_row || (dlen > 4u * G && isz > 2u * G);
+    defer_row = defer_row || (dlen > 16u * G && isz > 4u * G);
   }
   uint64_t kids = 0, leaves = 0;
   for (uint32_t t0 = 0; t0 < dlen; t0 += G) {
