@@ -417,8 +417,9 @@ __device__ __forceinline__ uint64_t tail_row(const JoinLevel& L1, const CandEdge
     return leaves;
   }
   // A row whose shared intersection does not fit the tile, or that may have
-  // very many children (> 16G) while I is large (> 4G), is deferred: its children are counted here and written by the caller,
-  // which counts their leaves one row per group.  Other rows stage I in LDS once and count each
+  // very many children (> 16G) while I is large (> 4G), is deferred: its
+  // children are counted here and written by the caller, which counts their
+  // leaves one row per group.  Other rows stage I in LDS once and count each
   // child's leaves cooperatively: lanes split the shorter of I and the
   // child's own list and search the other.
   bool defer_row = false;
