@@ -5,17 +5,23 @@
 // pair) descriptors, so the whole batch costs a fixed number of launches and
 // host round trips instead of a dozen per query.
 //
-// The join is one persistent kernel fed by a device work queue of
-// (query, root range) tasks, launched through a captured hipGraph.  Each
-// group of G lanes pops a task and walks its roots depth-first, keeping one
-// candidate list per level on a private stack in HBM; the last level is
-// counted, never stored.  No host synchronisation happens between levels or
-// queries, which is where the per-query plain path spends most of its time
-// on small queries (it syncs per level to size the next table).
+// Two join engines:
+//  * lockstep: level p of every query in one count launch, one scan and one
+//    write launch over all queries' rows (the row's query is looked up by
+//    binary search over the row bases); the last two levels of a query are
+//    counted by the fused tail in its count launch.  The host syncs once per
+//    level of the batch instead of once per level of every query.
+//  * persistent: one kernel fed by a device work queue of (query, root
+//    range) tasks, launched through a captured hipGraph.  Each group of G
+//    lanes pops a task and walks its roots depth-first, keeping one
+//    candidate list per level on a private stack in HBM, and donates
+//    unexplored ranges to idle groups; the last level is counted, never
+//    stored.  It needs no level tables, so it also runs batches whose
+//    levels do not fit the arena.
 //
-// The autotuner times the persistent kernel over a sample of the task queue
-// for each candidate (lanes per group, intersection strategy, LDS tile) and
-// runs the batch with the fastest; choices are cached per batch shape.
+// The autotuner times the lockstep shapes on the whole batch and the
+// persistent walk (lanes per group, LDS staging) on a sample, and runs the
+// batch with the fastest; choices are cached per batch shape.
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -225,71 +231,6 @@ struct BPair {
 
 __global__ void k_bscatter_pos(const uint32_t* __restrict__ cand, uint32_t n, uint32_t* __restrict__ pos) {
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) pos[cand[j]] = j;
-}
-
-template <int G>
-__global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_count(const BPair* __restrict__ P, uint32_t np, uint64_t total,
-                                                             const uint32_t* __restrict__ off,
-                                                             const uint32_t* __restrict__ adj,
-                                                             uint32_t* __restrict__ cnt,
-                                                             unsigned int* __restrict__ maxlen) {
-  constexpr int GPB = FSIG_BLOCK / G;
-  const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
-  const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
-  const uint64_t ngroups = (uint64_t)gridDim.x * GPB;
-  for (uint64_t f = (uint64_t)xcd_block() * GPB + gw; f < total; f += ngroups) {
-    const uint32_t pi = find_flat(P, np, f);
-    const BPair& pd = P[pi];
-    const uint32_t v = pd.acand[f - pd.flat];
-    const uint32_t s = off[v], e = off[v + 1];
-    uint32_t got = 0;
-    for (uint32_t j = s; j < e; j += G) {
-      const uint32_t i = j + gl;
-      const bool hit = i < e && ((pd.mask[adj[i]] >> pd.bbit) & 1u);
-      got += __popcll(__ballot(hit) & gmask);
-    }
-    if (gl == 0) {
-      cnt[f] = got;
-      atomicMax(&maxlen[pi], got);
-    }
-  }
-}
-
-template <int G>
-__global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_fill(const BPair* __restrict__ P, uint32_t np, uint64_t total,
-                                                            const uint32_t* __restrict__ off,
-                                                            const uint32_t* __restrict__ adj,
-                                                            const uint64_t* __restrict__ gscan,
-                                                            uint32_t* __restrict__ lst_all) {
-  constexpr int GPB = FSIG_BLOCK / G;
-  const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
-  const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
-  const uint64_t below = (1ull << lane) - 1;
-  const uint64_t ngroups = (uint64_t)gridDim.x * GPB;
-  for (uint64_t f = (uint64_t)xcd_block() * GPB + gw; f < total; f += ngroups) {
-    const BPair& pd = P[find_flat(P, np, f)];
-    const uint32_t i0 = (uint32_t)(f - pd.flat);
-    const uint32_t v = pd.acand[i0];
-    const uint64_t pbase = gscan[pd.flat];
-    uint64_t at = gscan[f];
-    if (gl == 0) {
-      pd.off_out[i0] = (uint32_t)(at - pbase);
-      if (i0 + 1 == pd.asize) pd.off_out[pd.asize] = (uint32_t)(gscan[f + 1] - pbase);
-    }
-    const uint32_t s = off[v], e = off[v + 1];
-    for (uint32_t j = s; j < e; j += G) {
-      const uint32_t i = j + gl;
-      uint32_t x = 0;
-      bool hit = false;
-      if (i < e) {
-        x = adj[i];
-        hit = (pd.mask[x] >> pd.bbit) & 1u;
-      }
-      const uint64_t bal = __ballot(hit) & gmask;
-      if (hit) lst_all[at + __popcll(bal & below)] = pd.bpos ? pd.bpos[x] : sorted_index(pd.bcand, pd.bsize, x);
-      at += __popcll(bal);
-    }
-  }
 }
 
 __global__ void k_bpair_base(const uint64_t* __restrict__ gscan, const BPair* __restrict__ P, uint32_t np,

@@ -102,14 +102,17 @@ int fsig_match(fsig_ctx* ctx, const fsig_graph* g, uint32_t k, const int32_t* ql
                uint64_t max_embeddings, uint64_t* count_out, uint32_t* emb_out, uint64_t emb_cap);
 
 /* Counts the embeddings of nq queries (alg 1 semantics) against one
- * resident graph.  Filter, refinement and candidate edges run for the whole
- * batch at once; the join is one persistent kernel, launched as a captured
- * hipGraph, whose groups take (query, root range) tasks from a device work
- * queue and walk them depth-first.  An autotuner times the candidate
- * configurations (lanes per group, LDS staging, tile) on a sample of the
- * queue unless fsig_set_tune fixed lanes_per_row.  counts_out[i] is query
- * i's count; *join_ms (may be NULL) the persistent kernel's device time.
- * No counterpart in the reference (it runs one query per process). */
+ * resident graph.  Filter (with the neighbour-label filter), refinement and
+ * candidate edges run for the whole batch at once.  The join is either the
+ * lockstep batched join (every query's level p in one count / scan / write
+ * sequence, the last two levels of count queries fused) or one persistent
+ * kernel, launched as a captured hipGraph, whose lane groups take
+ * (query, root range) tasks from a device work queue and walk them
+ * depth-first.  An autotuner times the engines and their lane counts and
+ * LDS staging (lockstep shapes on the whole batch, the walk on a sample)
+ * unless fsig_set_tune fixes them; the choice is cached per batch shape.
+ * counts_out[i] is query i's count; *join_ms (may be NULL) the join's device
+ * time.  No counterpart in the reference (it runs one query per process). */
 int fsig_match_batch(fsig_ctx* ctx, const fsig_graph* g, uint32_t nq, const uint32_t* k,
                      const int32_t* const* qlabels, const uint32_t* const* qoff,
                      const uint32_t* const* qadj, uint64_t* counts_out, double* join_ms);
