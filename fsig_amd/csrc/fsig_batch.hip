@@ -251,6 +251,10 @@ struct BSrc {
   uint32_t pidx[MAXQ];  // pair index of target b
 };
 
+// Candidates of more than kHubDeg neighbours are swept by k_bcedge_hub (a
+// workgroup per candidate) instead of one lane group.
+constexpr uint32_t kHubDeg = 2048;
+
 template <int G>
 __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_count_multi(const BSrc* __restrict__ S, uint32_t ns,
                                                                    uint64_t total, const BPair* __restrict__ P,
@@ -267,9 +271,10 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_count_multi(const BSrc* _
     const uint32_t i0 = (uint32_t)(f - sc.flat);
     const uint32_t v = sc.acand[i0];
     const uint32_t tm = sc.tmask;
+    const uint32_t s = off[v], e = off[v + 1];
+    if (e - s > kHubDeg) continue;  // k_bcedge_hub
     if (gl == 0)
       for (uint32_t t = tm; t; t &= t - 1) acc[gw][__builtin_ctz(t)] = 0;
-    const uint32_t s = off[v], e = off[v + 1];
     for (uint32_t j = s; j < e; j += G) {
       const uint32_t i = j + gl;
       const uint32_t m = i < e ? (sc.mask[adj[i]] & tm) : 0u;
@@ -305,6 +310,8 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_fill_multi(const BSrc* __
     const uint32_t i0 = (uint32_t)(f - sc.flat);
     const uint32_t v = sc.acand[i0];
     const uint32_t tm = sc.tmask;
+    const uint32_t s = off[v], e = off[v + 1];
+    if (e - s > kHubDeg) continue;  // k_bcedge_hub
     if (gl == 0)
       for (uint32_t t = tm; t; t &= t - 1) {
         const uint32_t b = __builtin_ctz(t);
@@ -318,7 +325,6 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_fill_multi(const BSrc* __
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint32_t s = off[v], e = off[v + 1];
     for (uint32_t j = s; j < e; j += G) {
       const uint32_t i = j + gl;
       uint32_t x = 0, m = 0;
@@ -344,6 +350,111 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_fill_multi(const BSrc* __
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
+    }
+  }
+}
+
+// Candidate edges of hub candidates (degree > kHubDeg): the workgroup finds
+// the hubs among 256 candidates at a time, then sweeps each hub's adjacency
+// with its 4 wavefronts, one contiguous quarter each, so a hub costs
+// deg/256 steps instead of deg/G on one lane group.  FILL == false: counts
+// per target class; FILL == true: each wavefront writes its quarter's
+// entries after the earlier quarters', so lists stay ascending.
+template <bool FILL>
+__global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_hub(const BSrc* __restrict__ S, uint32_t ns, uint64_t total,
+                                                           const BPair* __restrict__ P,
+                                                           const uint32_t* __restrict__ off,
+                                                           const uint32_t* __restrict__ adj,
+                                                           uint32_t* __restrict__ cnt,
+                                                           const uint64_t* __restrict__ gscan,
+                                                           uint32_t* __restrict__ lst_all) {
+  constexpr uint32_t W = FSIG_BLOCK / 64;
+  __shared__ uint32_t hubs[FSIG_BLOCK];
+  __shared__ uint32_t nh;
+  __shared__ uint32_t wcnt[W][MAXQ];
+  __shared__ uint64_t wat[W][MAXQ];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint64_t below = (1ull << lane) - 1;
+  for (uint64_t base = (uint64_t)blockIdx.x * FSIG_BLOCK; base < total; base += (uint64_t)gridDim.x * FSIG_BLOCK) {
+    if (threadIdx.x == 0) nh = 0;
+    __syncthreads();
+    const uint64_t f = base + threadIdx.x;
+    if (f < total) {
+      const BSrc& sc = S[find_flat(S, ns, f)];
+      const uint32_t v = sc.acand[f - sc.flat];
+      if (off[v + 1] - off[v] > kHubDeg) hubs[atomicAdd(&nh, 1u)] = threadIdx.x;
+    }
+    __syncthreads();
+    const uint32_t n_h = nh;
+    for (uint32_t h = 0; h < n_h; h++) {
+      const uint64_t fh = base + hubs[h];
+      const BSrc& sc = S[find_flat(S, ns, fh)];
+      const uint32_t i0 = (uint32_t)(fh - sc.flat);
+      const uint32_t v = sc.acand[i0];
+      const uint32_t tm = sc.tmask;
+      const uint32_t s = off[v], len = off[v + 1] - s, q = (len + W - 1) / W;
+      const uint32_t ws = s + min(len, wave * q), we = s + min(len, (wave + 1) * q);
+      if (lane == 0)
+        for (uint32_t t = tm; t; t &= t - 1) wcnt[wave][__builtin_ctz(t)] = 0;
+      lds_wave_sync();
+      for (uint32_t j = ws; j < we; j += 64) {
+        const uint32_t i = j + lane;
+        const uint32_t m = i < we ? (sc.mask[adj[i]] & tm) : 0u;
+        for (uint32_t t = tm; t; t &= t - 1) {
+          const uint32_t b = __builtin_ctz(t);
+          const uint32_t got = (uint32_t)__popcll(__ballot((m >> b) & 1u));
+          if (lane == 0) wcnt[wave][b] += got;
+        }
+      }
+      __syncthreads();
+      if (!FILL) {
+        if (threadIdx.x == 0)
+          for (uint32_t t = tm; t; t &= t - 1) {
+            const uint32_t b = __builtin_ctz(t);
+            uint32_t sum = 0;
+            for (uint32_t w = 0; w < W; w++) sum += wcnt[w][b];
+            cnt[P[sc.pidx[b]].flat + i0] = sum;
+          }
+      } else {
+        if (lane == 0)
+          for (uint32_t t = tm; t; t &= t - 1) {
+            const uint32_t b = __builtin_ctz(t);
+            const BPair& pd = P[sc.pidx[b]];
+            const uint64_t a0 = gscan[pd.flat + i0];
+            uint64_t pre = 0;
+            for (uint32_t w = 0; w < wave; w++) pre += wcnt[w][b];
+            wat[wave][b] = a0 + pre;
+            if (wave == 0) {
+              const uint64_t pbase = gscan[pd.flat];
+              pd.off_out[i0] = (uint32_t)(a0 - pbase);
+              if (i0 + 1 == pd.asize) pd.off_out[pd.asize] = (uint32_t)(gscan[pd.flat + i0 + 1] - pbase);
+            }
+          }
+        lds_wave_sync();
+        for (uint32_t j = ws; j < we; j += 64) {
+          const uint32_t i = j + lane;
+          uint32_t x = 0, m = 0;
+          if (i < we) {
+            x = adj[i];
+            m = sc.mask[x] & tm;
+          }
+          for (uint32_t t = tm; t; t &= t - 1) {
+            const uint32_t b = __builtin_ctz(t);
+            const bool hit = (m >> b) & 1u;
+            const uint64_t bal = __ballot(hit);
+            if (bal == 0) continue;
+            const uint64_t a = wat[wave][b];
+            if (hit) {
+              const BPair& pd = P[sc.pidx[b]];
+              lst_all[a + __popcll(bal & below)] = pd.bpos ? pd.bpos[x] : sorted_index(pd.bcand, pd.bsize, x);
+            }
+            lds_wave_sync();
+            if (lane == 0) wat[wave][b] = a + __popcll(bal);
+            lds_wave_sync();
+          }
+        }
+      }
+      __syncthreads();
     }
   }
 }
@@ -1449,6 +1560,11 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
         default: k_bcedge_count_multi<64><<<bgrid(c, sflat, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, cnt); break;
       }
     }
+    const uint32_t hub_grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((sflat + FSIG_BLOCK - 1) / FSIG_BLOCK,
+                                                                                 (uint64_t)c->num_cus * 8));
+    if (sflat)
+      k_bcedge_hub<false><<<hub_grid, FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, cnt,
+                                                                  nullptr, nullptr);
     FSIG_HIP(c, hipGetLastError());
     if ((rc = scan_u32_to_u64(c, cnt, gscan, pflat))) return rc;
     uint64_t total_edges;
@@ -1462,6 +1578,9 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
         default: k_bcedge_fill_multi<64><<<bgrid(c, sflat, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, gscan, lst_all); break;
       }
     }
+    if (sflat)
+      k_bcedge_hub<true><<<hub_grid, FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, nullptr,
+                                                                 gscan, lst_all);
     k_bpair_maxlen<<<np, FSIG_BLOCK, 0, c->stream>>>(dP, gscan, dmax);
     k_bpair_base<<<(np + 255) / 256, 256, 0, c->stream>>>(gscan, dP, np, pb);
     FSIG_HIP(c, hipGetLastError());

@@ -90,3 +90,20 @@ def test_batch_lanes_per_query(gpu_ctx):
     finally:
         gpu_ctx.set_tune()
     assert counts.tolist() == list(ref)
+
+
+def test_batch_hub_candidates(gpu_ctx):
+    # R-MAT hubs (max degree 3201 > 2048): their candidate-edge lists come
+    # from the per-hub workgroup sweep; counts must equal the plain path's
+    g = F.HostGraph.rmat(13, 200000, seed=21).set_labels("uniform", 3, seed=22)
+    assert g.max_degree > 2048
+    dg = gpu_ctx.upload(g)
+    qs, ref = zip(*_bounded(dg, [q for _, q in W.walk_queries(g, 24, 3, 6, 23)], 20_000_000))
+    for strategy in (3, 1):  # lockstep and persistent joins over the same lists
+        try:
+            gpu_ctx.set_tune(lanes_per_row=16 if strategy == 1 else 0, strategy=strategy)
+            counts, _ = F.match_batch(dg, qs)
+        finally:
+            gpu_ctx.set_tune()
+        assert counts.tolist() == list(ref), strategy
+    dg.free()
