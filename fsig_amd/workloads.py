@@ -152,7 +152,7 @@ def c5_graph(scale: int = 24, m: int = 200_000_000, seed: int = 8) -> HostGraph:
 # from their own seed).  Drawing the stream checks every candidate on the
 # device, so a drawn stream is kept in a JSON file: each query with the walk
 # seed that reproduces it.
-C5_STREAM = dict(count=128, kmin=4, kmax=32, k_cap=5, seed=11, arrival_seed=12, qps=400.0, max_count=10 ** 8,
+C5_STREAM = dict(count=128, kmin=4, kmax=32, k_cap=5, seed=11, arrival_seed=12, qps=1000.0, max_count=10 ** 8,
                  max_rows=2 * 10 ** 7, max_attempts=200)
 C5_STREAM_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "c5_stream.json")
 

@@ -34,7 +34,7 @@ def test_c5_stream_file_matches_parameters():
 
 def test_c5_arrivals_are_seeded_poisson():
     # the arrival process depends only on its seed and the rate
-    g = np.random.default_rng(W.C5_STREAM["arrival_seed"]).exponential(1.0 / 400.0, size=128)
+    g = np.random.default_rng(W.C5_STREAM["arrival_seed"]).exponential(1.0 / 1000.0, size=128)
     t = np.cumsum(g) - g[0]
     assert t[0] == 0 and np.all(np.diff(t) > 0)
-    assert 0.2 < t[-1] < 0.45  # 127 gaps of mean 2.5 ms
+    assert 0.08 < t[-1] < 0.18  # 127 gaps of mean 1 ms
