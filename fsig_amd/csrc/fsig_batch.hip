@@ -1145,10 +1145,6 @@ __global__ void k_gather_bases(const uint64_t* __restrict__ off, const uint64_t*
   if (i <= n) out[i] = off[qbase[i]];
 }
 
-__global__ void k_zero_u32(unsigned int* p, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
 
 // Resets the work queue for a run over `seeds` seed chunks (a kernel, so the
 // reset can live inside the captured graph).
