@@ -96,6 +96,15 @@ def host_lib():
         L.fsig_murmur2.restype = C.c_uint32
         L.fsig_murmur2.argtypes = [C.c_void_p, C.c_int, C.c_uint32]
         L.fsig_write_embeddings.argtypes = [C.c_char_p, C.c_uint32, C.c_uint64, _u32p, C.c_uint64]
+        L.fsig_hg_save_bin.argtypes = [_vp, C.c_char_p]
+        L.fsig_hg_load_bin.restype = _vp
+        L.fsig_hg_load_bin.argtypes = [C.c_char_p, C.POINTER(C.c_int)]
+        L.fsig_pcsr_save_bin.argtypes = [_vp, _vp, C.c_char_p]
+        L.fsig_pcsr_load_bin.restype = _vp
+        L.fsig_pcsr_load_bin.argtypes = [C.c_char_p, C.POINTER(C.c_int)]
+        L.fsig_pcsr_graph_info.argtypes = [_vp, _u32p, _u32p, _i32p]
+        L.fsig_pcsr_labels.restype = _i32p
+        L.fsig_pcsr_labels.argtypes = [_vp]
         _host = L
     return _host
 
@@ -208,6 +217,19 @@ class HostGraph:
         return cls(h)
 
     @classmethod
+    def load_bin(cls, path: str) -> "HostGraph":
+        """A graph from its binary CSR cache (fsig_hg_load_bin)."""
+        st = C.c_int(0)
+        h = host_lib().fsig_hg_load_bin(path.encode(), C.byref(st))
+        if not h:
+            raise FsigError(st.value, f"cannot load the CSR cache {path}")
+        return cls(h)
+
+    def save_bin(self, path: str) -> None:
+        if self._L.fsig_hg_save_bin(self._h, path.encode()):
+            raise FsigError(FSIG_EINVAL, f"cannot write {path}")
+
+    @classmethod
     def from_edges(cls, n: int, edges, labels=None) -> "HostGraph":
         e = np.ascontiguousarray(np.asarray(edges, dtype=np.uint32).reshape(-1, 2))
         lab = None if labels is None else np.ascontiguousarray(np.asarray(labels, dtype=np.int32))
@@ -284,6 +306,29 @@ class HostPCSR:
             self._L.fsig_pcsr_free(self._h)
         except Exception:
             pass
+
+    @classmethod
+    def load_bin(cls, path: str, reference_unit_ids: bool = False) -> "HostPCSR":
+        """Units (and the graph header and labels) from a PCSR cache in the
+        reference's layout; reference_unit_ids states how its units are
+        numbered."""
+        st = C.c_int(0)
+        h = host_lib().fsig_pcsr_load_bin(path.encode(), C.byref(st))
+        if not h:
+            raise FsigError(st.value, f"cannot load the PCSR cache {path}")
+        return cls(h, reference_unit_ids)
+
+    def save_bin(self, path: str, graph: "HostGraph") -> None:
+        if self._L.fsig_pcsr_save_bin(self._h, graph._h, path.encode()):
+            raise FsigError(FSIG_EINVAL, f"cannot write {path}")
+
+    def graph_info(self) -> Tuple[int, int, int, np.ndarray]:
+        """(vertices, undirected edges, labels, label array) of a loaded cache."""
+        n, m, nl = C.c_uint32(), C.c_uint32(), C.c_int32()
+        self._L.fsig_pcsr_graph_info(self._h, C.byref(n), C.byref(m), C.byref(nl))
+        lab = np.ctypeslib.as_array(self._L.fsig_pcsr_labels(self._h), shape=(n.value,)).copy() \
+            if n.value else np.empty(0, np.int32)
+        return n.value, m.value, nl.value, lab
 
     def unit(self, u: int) -> Tuple[np.ndarray, np.ndarray]:
         keys, ne = C.c_uint32(), C.c_uint32()

@@ -1,9 +1,10 @@
 // C ABI of libfsig_host.so (include/fsig_host.h).
-#include <mutex>
 #include "../../include/fsig_host.h"
 
 #include <algorithm>
+#include <cstdio>
 #include <memory>
+#include <mutex>
 
 #include "gen.h"
 #include "graph.h"
@@ -26,7 +27,37 @@ struct fsig_hgraph {
 
 struct fsig_hpcsr {
   std::vector<PCSRUnit> units;
+  // graph header and labels when the units come from a cache file
+  uint32_t n = 0, num_edges = 0;
+  int32_t num_labels = 0;
+  std::vector<int32_t> labels;
 };
+
+// Binary caches.  Every field is 4 bytes, native byte order, no padding.
+//   PCSR (the reference's commented STORE / READ blocks,
+//   graph_format_transformer/graph_format_transformer.cc:47-74, 197-213):
+//     u32 num_vertices, u32 num_edges, i32 num_labels, i32 labels[num_vertices],
+//     u32 num_units, then per unit: u32 num_keys, u32 num_edges,
+//     u32 offset[32 * num_keys], u32 edges[num_edges]
+//   CSR (the same header): u32 num_vertices, u32 num_edges (undirected),
+//     i32 num_labels, i32 labels[num_vertices], u32 offset[num_vertices + 1],
+//     u32 adjacency[2 * num_edges]
+namespace {
+
+struct BinFile {
+  FILE* f = nullptr;
+  explicit BinFile(const char* path, const char* mode) : f(std::fopen(path, mode)) {}
+  ~BinFile() {
+    if (f) std::fclose(f);
+  }
+  template <class T>
+  bool put(const T* p, size_t n) { return n == 0 || std::fwrite(p, sizeof(T), n, f) == n; }
+  template <class T>
+  bool get(T* p, size_t n) { return n == 0 || std::fread(p, sizeof(T), n, f) == n; }
+  bool at_end() { return std::fgetc(f) == EOF; }
+};
+
+}  // namespace
 
 static void fix_num_labels(fsig_hgraph* g) {
   int32_t mx = 0;
@@ -183,6 +214,87 @@ void fsig_pcsr_unit_info(const fsig_hpcsr* p, uint32_t u, uint32_t* keys, uint32
 const uint32_t* fsig_pcsr_unit_offset(const fsig_hpcsr* p, uint32_t u) { return p->units[u].offset_.data(); }
 const uint32_t* fsig_pcsr_unit_edges(const fsig_hpcsr* p, uint32_t u) { return p->units[u].edges_.data(); }
 void fsig_pcsr_free(fsig_hpcsr* p) { delete p; }
+
+int fsig_hg_save_bin(const fsig_hgraph* g, const char* path) {
+  if (!g || !path) return -1;
+  BinFile b(path, "wb");
+  if (!b.f) return -1;
+  const uint32_t m = g->off[g->n] / 2;
+  const bool ok = b.put(&g->n, 1) && b.put(&m, 1) && b.put(&g->num_labels, 1) && b.put(g->labels.data(), g->n) &&
+                  b.put(g->off.data(), g->n + 1ull) && b.put(g->adj.data(), g->adj.size());
+  return ok ? 0 : -1;
+}
+
+fsig_hgraph* fsig_hg_load_bin(const char* path, int* status) {
+  auto fail = [&](int code) -> fsig_hgraph* {
+    if (status) *status = code;
+    return nullptr;
+  };
+  BinFile b(path, "rb");
+  if (!b.f) return fail(kIoCannotOpen);
+  auto g = std::make_unique<fsig_hgraph>();
+  uint32_t m = 0;
+  if (!b.get(&g->n, 1) || !b.get(&m, 1) || !b.get(&g->num_labels, 1)) return fail(kIoBadHeader);
+  g->labels.resize(g->n);
+  g->off.resize(g->n + 1ull);
+  if (!b.get(g->labels.data(), g->n) || !b.get(g->off.data(), g->n + 1ull)) return fail(kIoBadRecord);
+  if (g->off[0] != 0 || g->off[g->n] != 2ull * m) return fail(kIoBadRecord);
+  g->adj.resize(2ull * m);
+  if (!b.get(g->adj.data(), g->adj.size()) || !b.at_end()) return fail(kIoBadRecord);
+  for (uint32_t v = 0; v < g->n; v++)
+    if (g->off[v + 1] < g->off[v] || g->labels[v] < 0 || g->labels[v] >= g->num_labels) return fail(kIoBadRecord);
+  for (uint32_t x : g->adj)
+    if (x >= g->n) return fail(kIoEdgeOutOfRange);
+  if (status) *status = 0;
+  return g.release();
+}
+
+int fsig_pcsr_save_bin(const fsig_hpcsr* p, const fsig_hgraph* g, const char* path) {
+  if (!p || !g || !path) return -1;
+  BinFile b(path, "wb");
+  if (!b.f) return -1;
+  const uint32_t m = g->off[g->n] / 2, nu = (uint32_t)p->units.size();
+  bool ok = b.put(&g->n, 1) && b.put(&m, 1) && b.put(&g->num_labels, 1) && b.put(g->labels.data(), g->n) &&
+            b.put(&nu, 1);
+  for (const PCSRUnit& u : p->units)
+    ok = ok && b.put(&u.num_keys_, 1) && b.put(&u.num_edges_, 1) && b.put(u.offset_.data(), 32ull * u.num_keys_) &&
+         b.put(u.edges_.data(), u.num_edges_);
+  return ok ? 0 : -1;
+}
+
+fsig_hpcsr* fsig_pcsr_load_bin(const char* path, int* status) {
+  auto fail = [&](int code) -> fsig_hpcsr* {
+    if (status) *status = code;
+    return nullptr;
+  };
+  BinFile b(path, "rb");
+  if (!b.f) return fail(kIoCannotOpen);
+  auto p = std::make_unique<fsig_hpcsr>();
+  uint32_t nu = 0;
+  if (!b.get(&p->n, 1) || !b.get(&p->num_edges, 1) || !b.get(&p->num_labels, 1)) return fail(kIoBadHeader);
+  p->labels.resize(p->n);
+  if (!b.get(p->labels.data(), p->n) || !b.get(&nu, 1)) return fail(kIoBadRecord);
+  if (p->num_labels < 1 || nu != (uint64_t)p->num_labels * (p->num_labels + 1) / 2) return fail(kIoBadHeader);
+  p->units.resize(nu);
+  for (PCSRUnit& u : p->units) {
+    if (!b.get(&u.num_keys_, 1) || !b.get(&u.num_edges_, 1)) return fail(kIoBadRecord);
+    u.offset_.resize(32ull * u.num_keys_);
+    u.edges_.resize(u.num_edges_);
+    if (!b.get(u.offset_.data(), u.offset_.size()) || !b.get(u.edges_.data(), u.edges_.size()))
+      return fail(kIoBadRecord);
+  }
+  if (!b.at_end()) return fail(kIoBadRecord);
+  if (status) *status = 0;
+  return p.release();
+}
+
+void fsig_pcsr_graph_info(const fsig_hpcsr* p, uint32_t* n, uint32_t* num_edges, int32_t* num_labels) {
+  if (n) *n = p->n;
+  if (num_edges) *num_edges = p->num_edges;
+  if (num_labels) *num_labels = p->num_labels;
+}
+
+const int32_t* fsig_pcsr_labels(const fsig_hpcsr* p) { return p->labels.data(); }
 
 uint32_t fsig_murmur2(const void* key, int len, uint32_t seed) { return murmur_hash2(key, len, seed); }
 

@@ -2,12 +2,14 @@
 // Same positional arguments as main/run.cc:27-64; argv[5], which the
 // reference accepts but never writes, receives the canonical embedding file
 // (`#embedding: <count>` then the sorted rows, query-vertex order) when given;
-// argv[7] bounds the enumeration (0 = all).
+// argv[7] bounds the enumeration (0 = all).  A data graph named *.csr.bin is
+// read from the binary CSR cache instead of the text format.
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
 #include <string>
 
+#include "../../include/fsig_host.h"
 #include "graph.h"
 #include "io.h"
 #include "matcher.h"
@@ -33,15 +35,47 @@ int main(int argc, const char* argv[]) {
   }
   TimeRecorder tr;
   UndirectedPlainGraph data, query;
+  // a data graph ending in .csr.bin is the binary CSR cache (alg 1 only: the
+  // GSI path builds its signatures from the plain graph)
+  const std::string bin_suffix = ".csr.bin";
+  const bool from_bin = data_file.size() > bin_suffix.size() &&
+                        data_file.compare(data_file.size() - bin_suffix.size(), bin_suffix.size(), bin_suffix) == 0;
+  if (from_bin && alg != 1) {
+    printf("invalid arguments! a binary CSR cache serves algorithm 1 only\n");
+    return -1;
+  }
+  std::unique_ptr<UndirectedCSRGraph> cached;
+  int rc = 0;
   tr.event_start("read data");
-  int rc = read_plain_graph(data_file, labeled, data);
+  if (from_bin) {
+    fsig_hgraph* hg = fsig_hg_load_bin(data_file.c_str(), &rc);
+    if (hg) {
+      uint32_t n = 0, md = 0;
+      uint64_t nnz = 0;
+      int32_t nl = 0;
+      fsig_hg_info(hg, &n, &nnz, &nl, &md);
+      cached.reset(new UndirectedCSRGraph());
+      cached->num_vertices_ = n;
+      cached->num_edges_ = (Edge)(nnz / 2);
+      cached->num_labels_ = nl;
+      cached->offset_.assign(fsig_hg_off(hg), fsig_hg_off(hg) + n + 1);
+      cached->edges_.assign(fsig_hg_adj(hg), fsig_hg_adj(hg) + nnz);
+      cached->labels_.assign(fsig_hg_labels(hg), fsig_hg_labels(hg) + n);
+      fsig_hg_free(hg);
+    }
+  } else {
+    rc = read_plain_graph(data_file, labeled, data);
+  }
   tr.event_end("read data");
   if (rc) { printf("%s\n", io_status_string(rc)); return 1; }
   tr.event_start("read query");
   rc = read_plain_graph(query_file, labeled, query);
   tr.event_end("read query");
   if (rc) { printf("%s\n", io_status_string(rc)); return 1; }
-  printf("|V|: %u\n|E|: %u\n|L|: %d\n", data.num_vertices_, data.num_edges_, data.num_labels_);
+  if (cached)
+    printf("|V|: %u\n|E|: %u\n|L|: %d\n", cached->num_vertices_, cached->num_edges_, cached->num_labels_);
+  else
+    printf("|V|: %u\n|E|: %u\n|L|: %d\n", data.num_vertices_, data.num_edges_, data.num_labels_);
   printf("|V|: %u\n|E|: %u\n|L|: %d\n", query.num_vertices_, query.num_edges_, query.num_labels_);
 
   Graph* organised = nullptr;
@@ -49,7 +83,8 @@ int main(int argc, const char* argv[]) {
   std::unique_ptr<GraphFormatTransformer> t;
   if (alg == 1) t.reset(new UndirectedPlainToCSRTransformer(true));
   else t.reset(new UndirectedPlainToPCSRTransformer(false));
-  rc = t->transform(&data, organised);
+  if (cached) organised = cached.release();
+  else rc = t->transform(&data, organised);
   tr.event_end("organise graph");
   if (rc || !organised) { printf("Error: transform plain graph\n"); return 1; }
   std::unique_ptr<Graph> owned(organised);

@@ -57,6 +57,24 @@ const uint32_t* fsig_pcsr_unit_offset(const fsig_hpcsr* p, uint32_t u);
 const uint32_t* fsig_pcsr_unit_edges(const fsig_hpcsr* p, uint32_t u);
 void fsig_pcsr_free(fsig_hpcsr* p);
 
+/* Binary caches, all fields 4 bytes in native order.  PCSR: the layout of
+ * the reference's commented STORE / READ blocks
+ * (graph_format_transformer/graph_format_transformer.cc:47-74, 197-213):
+ * num_vertices, num_edges, num_labels, labels[num_vertices], num_units, then
+ * per unit num_keys, num_edges, offset[32*num_keys], edges[num_edges].  A
+ * loaded PCSR carries the header and labels (fsig_pcsr_graph_info /
+ * fsig_pcsr_labels).  CSR: the same header and labels, then offset[n+1] and
+ * the 2*num_edges adjacency entries.  Loads return NULL with *status one of
+ * the reader's error codes on a missing, short, inconsistent or overlong
+ * file.  Replace re-parsing the text format (the reference's cache blocks
+ * are commented out). */
+int fsig_hg_save_bin(const fsig_hgraph* g, const char* path);
+fsig_hgraph* fsig_hg_load_bin(const char* path, int* status);
+int fsig_pcsr_save_bin(const fsig_hpcsr* p, const fsig_hgraph* g, const char* path);
+fsig_hpcsr* fsig_pcsr_load_bin(const char* path, int* status);
+void fsig_pcsr_graph_info(const fsig_hpcsr* p, uint32_t* n, uint32_t* num_edges, int32_t* num_labels);
+const int32_t* fsig_pcsr_labels(const fsig_hpcsr* p);
+
 uint32_t fsig_murmur2(const void* key, int len, uint32_t seed);
 
 /* Canonical embedding file: `#embedding: <count>` then sorted rows. */
