@@ -6,6 +6,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <thread>
 
 namespace fsig {
 
@@ -71,6 +72,42 @@ bool slurp(const std::string& path, std::vector<char>& buf) {
 
 }  // namespace
 
+namespace {
+
+// Records of one slice of the file body, in file order.
+struct ParsedSlice {
+  std::vector<std::pair<Vertex, Label>> vertices;
+  std::vector<std::pair<Vertex, Vertex>> edges;
+  int rc = kIoOk;
+};
+
+void parse_slice(const char* b, const char* e, bool is_labeled, ParsedSlice& out) {
+  Cursor c{b, e};
+  char type;
+  while (c.next_char(type)) {
+    if (type == 'v') {
+      uint64_t id, deg;
+      int64_t label;
+      if (!c.next_u64(id) || !c.next_i64(label) || !c.next_u64(deg)) { out.rc = kIoBadRecord; return; }
+      if (id >= kInvalid || label < 0 || label > 0x7fffffff) { out.rc = kIoBadRecord; return; }
+      out.vertices.emplace_back((Vertex)id, is_labeled ? (Label)label : 0);
+    } else if (type == 'e') {
+      uint64_t a, bb;
+      if (!c.next_u64(a) || !c.next_u64(bb)) { out.rc = kIoBadRecord; return; }
+      if (a >= kInvalid || bb >= kInvalid) { out.rc = kIoBadRecord; return; }
+      out.edges.emplace_back((Vertex)a, (Vertex)bb);
+    } else {
+      c.skip_line();  // unknown record kind: ignored, as the reference does
+    }
+  }
+}
+
+}  // namespace
+
+// Files above 16 MiB are parsed by up to 16 threads: the body is cut at line
+// starts of `v` / `e` records, every slice is parsed on its own and the
+// slices are applied in file order, so the result and the first error
+// reported are those of a sequential parse.
 int read_plain_graph(const std::string& path, bool is_labeled, UndirectedPlainGraph& g) {
   std::vector<char> buf;
   if (!slurp(path, buf)) return kIoCannotOpen;
@@ -81,34 +118,46 @@ int read_plain_graph(const std::string& path, bool is_labeled, UndirectedPlainGr
   if (!c.next_u64(hn) || !c.next_u64(hm)) return kIoBadHeader;
   g = UndirectedPlainGraph();
   g.is_labeled_ = is_labeled;
-  std::vector<std::pair<Vertex, Vertex>> edges;
-  edges.reserve((size_t)std::min<uint64_t>(hm, 1ull << 31));
-  while (c.next_char(type)) {
-    if (type == 'v') {
-      uint64_t id, deg;
-      int64_t label;
-      if (!c.next_u64(id) || !c.next_i64(label) || !c.next_u64(deg)) return kIoBadRecord;
-      if (id >= kInvalid || label < 0 || label > 0x7fffffff) return kIoBadRecord;
-      g.add_vertex((Vertex)id, is_labeled ? (Label)label : 0);
-    } else if (type == 'e') {
-      uint64_t a, b;
-      if (!c.next_u64(a) || !c.next_u64(b)) return kIoBadRecord;
-      if (a >= kInvalid || b >= kInvalid) return kIoBadRecord;
-      edges.emplace_back((Vertex)a, (Vertex)b);
-    } else {
-      c.skip_line();  // unknown record kind: ignored, as the reference does
-    }
+  const char* body = c.p;
+  const char* end = buf.data() + buf.size();
+  const size_t len = (size_t)(end - body);
+  unsigned T = 1;
+  if (len > (16u << 20)) T = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<const char*> cut{body};
+  for (unsigned t = 1; t < T; t++) {
+    const char* q = std::max(cut.back(), body + len / T * t);
+    while (q < end && !(q[0] == '\n' && q + 1 < end && (q[1] == 'v' || q[1] == 'e'))) q++;
+    cut.push_back(q);
   }
+  cut.push_back(end);
+  std::vector<ParsedSlice> slices(T);
+  if (T == 1) {
+    parse_slice(body, end, is_labeled, slices[0]);
+  } else {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; t++)
+      th.emplace_back(parse_slice, cut[t], cut[t + 1], is_labeled, std::ref(slices[t]));
+    for (auto& x : th) x.join();
+  }
+  for (auto& sl : slices)
+    if (sl.rc) return sl.rc;
+  for (auto& sl : slices)
+    for (auto& v : sl.vertices) g.add_vertex(v.first, v.second);
   for (Vertex v = 0; v < g.num_vertices_; v++)
     if (g.labels_[v] == kInvalidLabel) return kIoMissingVertex;
-  for (auto& e : edges)
-    if (e.first >= g.num_vertices_ || e.second >= g.num_vertices_) return kIoEdgeOutOfRange;
+  for (auto& sl : slices)
+    for (auto& e : sl.edges)
+      if (e.first >= g.num_vertices_ || e.second >= g.num_vertices_) return kIoEdgeOutOfRange;
   // Pre-size adjacency from degrees so huge files do not re-grow vectors.
   std::vector<uint32_t> deg(g.num_vertices_, 0);
-  for (auto& e : edges) { deg[e.first]++; deg[e.second]++; }
+  for (auto& sl : slices)
+    for (auto& e : sl.edges) { deg[e.first]++; deg[e.second]++; }
   g.edges_.assign(g.num_vertices_, {});
   for (Vertex v = 0; v < g.num_vertices_; v++) g.edges_[v].reserve(deg[v]);
-  for (auto& e : edges) g.add_edge(e.first, e.second);
+  for (auto& sl : slices) {
+    for (auto& e : sl.edges) g.add_edge(e.first, e.second);
+    std::vector<std::pair<Vertex, Vertex>>().swap(sl.edges);
+  }
   g.finalize();
   g.set_min_degree();
   return kIoOk;

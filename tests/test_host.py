@@ -155,3 +155,20 @@ def test_walk_query_is_induced_and_has_a_witness():
         for j in range(i + 1, 4):
             has = w[j] in set(adj[off[w[i]]:off[w[i] + 1]].tolist())
             assert has == ((i, j) in {tuple(e) for e in q.edges.tolist()})
+
+
+def test_large_file_parsed_in_slices(tmp_path):
+    # > 16 MiB: the reader parses slices in threads; the graph must equal a
+    # generated one, and an error in the last slice must still be reported
+    g = F.HostGraph.chung_lu(200000, 1500000, 2.3, 41).set_labels("zipf", 8, 1.2, 42)
+    p = str(tmp_path / "big.graph")
+    g.write(p)
+    assert os.path.getsize(p) > (16 << 20)
+    h = F.HostGraph.read(p)
+    for a, b in zip(g.arrays(), h.arrays()):
+        assert np.array_equal(a, b)
+    with open(p, "a") as f:
+        f.write("\ne 7 x")
+    with pytest.raises(F.FsigError) as e:
+        F.HostGraph.read(p)
+    assert e.value.code == -3  # truncated or malformed record
