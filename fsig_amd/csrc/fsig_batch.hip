@@ -354,14 +354,29 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_fill_multi(const BSrc* __
   }
 }
 
-// Candidate edges of hub candidates (degree > kHubDeg): the workgroup finds
-// the hubs among 256 candidates at a time, then sweeps each hub's adjacency
-// with its 4 wavefronts, one contiguous quarter each, so a hub costs
-// deg/256 steps instead of deg/G on one lane group.  FILL == false: counts
-// per target class; FILL == true: each wavefront writes its quarter's
-// entries after the earlier quarters', so lists stay ascending.
+// Hub candidates (degree > kHubDeg) of all sources, gathered into one list
+// so that k_bcedge_hub deals them out one per workgroup (hubs cluster at the
+// low vertex ids of power-law graphs, i.e. at the start of every class).
+__global__ void k_bfind_hubs(const BSrc* __restrict__ S, uint32_t ns, uint64_t total,
+                             const uint32_t* __restrict__ off, uint64_t* __restrict__ hubs,
+                             unsigned int* __restrict__ nhubs) {
+  for (uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; f < total;
+       f += (uint64_t)gridDim.x * blockDim.x) {
+    const BSrc& sc = S[find_flat(S, ns, f)];
+    const uint32_t v = sc.acand[f - sc.flat];
+    if (off[v + 1] - off[v] > kHubDeg) hubs[atomicAdd(nhubs, 1u)] = f;
+  }
+}
+
+// Candidate edges of one hub per workgroup: its 4 wavefronts sweep one
+// contiguous quarter of the adjacency each, so a hub costs deg/256 steps
+// instead of deg/G on one lane group.  FILL == false: counts per target
+// class; FILL == true: each wavefront writes its quarter's entries after the
+// earlier quarters', so lists stay ascending.
 template <bool FILL>
-__global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_hub(const BSrc* __restrict__ S, uint32_t ns, uint64_t total,
+__global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_hub(const BSrc* __restrict__ S, uint32_t ns,
+                                                           const uint64_t* __restrict__ hubs,
+                                                           const unsigned int* __restrict__ nhubs,
                                                            const BPair* __restrict__ P,
                                                            const uint32_t* __restrict__ off,
                                                            const uint32_t* __restrict__ adj,
@@ -369,93 +384,80 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_hub(const BSrc* __restric
                                                            const uint64_t* __restrict__ gscan,
                                                            uint32_t* __restrict__ lst_all) {
   constexpr uint32_t W = FSIG_BLOCK / 64;
-  __shared__ uint32_t hubs[FSIG_BLOCK];
-  __shared__ uint32_t nh;
   __shared__ uint32_t wcnt[W][MAXQ];
   __shared__ uint64_t wat[W][MAXQ];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint64_t below = (1ull << lane) - 1;
-  for (uint64_t base = (uint64_t)blockIdx.x * FSIG_BLOCK; base < total; base += (uint64_t)gridDim.x * FSIG_BLOCK) {
-    if (threadIdx.x == 0) nh = 0;
-    __syncthreads();
-    const uint64_t f = base + threadIdx.x;
-    if (f < total) {
-      const BSrc& sc = S[find_flat(S, ns, f)];
-      const uint32_t v = sc.acand[f - sc.flat];
-      if (off[v + 1] - off[v] > kHubDeg) hubs[atomicAdd(&nh, 1u)] = threadIdx.x;
+  const uint32_t n_h = *nhubs;
+  for (uint32_t h = blockIdx.x; h < n_h; h += gridDim.x) {
+    const uint64_t fh = hubs[h];
+    const BSrc& sc = S[find_flat(S, ns, fh)];
+    const uint32_t i0 = (uint32_t)(fh - sc.flat);
+    const uint32_t v = sc.acand[i0];
+    const uint32_t tm = sc.tmask;
+    const uint32_t s = off[v], len = off[v + 1] - s, q = (len + W - 1) / W;
+    const uint32_t ws = s + min(len, wave * q), we = s + min(len, (wave + 1) * q);
+    if (lane == 0)
+      for (uint32_t t = tm; t; t &= t - 1) wcnt[wave][__builtin_ctz(t)] = 0;
+    lds_wave_sync();
+    for (uint32_t j = ws; j < we; j += 64) {
+      const uint32_t i = j + lane;
+      const uint32_t m = i < we ? (sc.mask[adj[i]] & tm) : 0u;
+      for (uint32_t t = tm; t; t &= t - 1) {
+        const uint32_t b = __builtin_ctz(t);
+        const uint32_t got = (uint32_t)__popcll(__ballot((m >> b) & 1u));
+        if (lane == 0) wcnt[wave][b] += got;
+      }
     }
     __syncthreads();
-    const uint32_t n_h = nh;
-    for (uint32_t h = 0; h < n_h; h++) {
-      const uint64_t fh = base + hubs[h];
-      const BSrc& sc = S[find_flat(S, ns, fh)];
-      const uint32_t i0 = (uint32_t)(fh - sc.flat);
-      const uint32_t v = sc.acand[i0];
-      const uint32_t tm = sc.tmask;
-      const uint32_t s = off[v], len = off[v + 1] - s, q = (len + W - 1) / W;
-      const uint32_t ws = s + min(len, wave * q), we = s + min(len, (wave + 1) * q);
+    if (!FILL) {
+      if (threadIdx.x == 0)
+        for (uint32_t t = tm; t; t &= t - 1) {
+          const uint32_t b = __builtin_ctz(t);
+          uint32_t sum = 0;
+          for (uint32_t w = 0; w < W; w++) sum += wcnt[w][b];
+          cnt[P[sc.pidx[b]].flat + i0] = sum;
+        }
+    } else {
       if (lane == 0)
-        for (uint32_t t = tm; t; t &= t - 1) wcnt[wave][__builtin_ctz(t)] = 0;
+        for (uint32_t t = tm; t; t &= t - 1) {
+          const uint32_t b = __builtin_ctz(t);
+          const BPair& pd = P[sc.pidx[b]];
+          const uint64_t a0 = gscan[pd.flat + i0];
+          uint64_t pre = 0;
+          for (uint32_t w = 0; w < wave; w++) pre += wcnt[w][b];
+          wat[wave][b] = a0 + pre;
+          if (wave == 0) {
+            const uint64_t pbase = gscan[pd.flat];
+            pd.off_out[i0] = (uint32_t)(a0 - pbase);
+            if (i0 + 1 == pd.asize) pd.off_out[pd.asize] = (uint32_t)(gscan[pd.flat + i0 + 1] - pbase);
+          }
+        }
       lds_wave_sync();
       for (uint32_t j = ws; j < we; j += 64) {
         const uint32_t i = j + lane;
-        const uint32_t m = i < we ? (sc.mask[adj[i]] & tm) : 0u;
+        uint32_t x = 0, m = 0;
+        if (i < we) {
+          x = adj[i];
+          m = sc.mask[x] & tm;
+        }
         for (uint32_t t = tm; t; t &= t - 1) {
           const uint32_t b = __builtin_ctz(t);
-          const uint32_t got = (uint32_t)__popcll(__ballot((m >> b) & 1u));
-          if (lane == 0) wcnt[wave][b] += got;
-        }
-      }
-      __syncthreads();
-      if (!FILL) {
-        if (threadIdx.x == 0)
-          for (uint32_t t = tm; t; t &= t - 1) {
-            const uint32_t b = __builtin_ctz(t);
-            uint32_t sum = 0;
-            for (uint32_t w = 0; w < W; w++) sum += wcnt[w][b];
-            cnt[P[sc.pidx[b]].flat + i0] = sum;
-          }
-      } else {
-        if (lane == 0)
-          for (uint32_t t = tm; t; t &= t - 1) {
-            const uint32_t b = __builtin_ctz(t);
+          const bool hit = (m >> b) & 1u;
+          const uint64_t bal = __ballot(hit);
+          if (bal == 0) continue;
+          const uint64_t a = wat[wave][b];
+          if (hit) {
             const BPair& pd = P[sc.pidx[b]];
-            const uint64_t a0 = gscan[pd.flat + i0];
-            uint64_t pre = 0;
-            for (uint32_t w = 0; w < wave; w++) pre += wcnt[w][b];
-            wat[wave][b] = a0 + pre;
-            if (wave == 0) {
-              const uint64_t pbase = gscan[pd.flat];
-              pd.off_out[i0] = (uint32_t)(a0 - pbase);
-              if (i0 + 1 == pd.asize) pd.off_out[pd.asize] = (uint32_t)(gscan[pd.flat + i0 + 1] - pbase);
-            }
+            lst_all[a + __popcll(bal & below)] = pd.bpos ? pd.bpos[x] : sorted_index(pd.bcand, pd.bsize, x);
           }
-        lds_wave_sync();
-        for (uint32_t j = ws; j < we; j += 64) {
-          const uint32_t i = j + lane;
-          uint32_t x = 0, m = 0;
-          if (i < we) {
-            x = adj[i];
-            m = sc.mask[x] & tm;
-          }
-          for (uint32_t t = tm; t; t &= t - 1) {
-            const uint32_t b = __builtin_ctz(t);
-            const bool hit = (m >> b) & 1u;
-            const uint64_t bal = __ballot(hit);
-            if (bal == 0) continue;
-            const uint64_t a = wat[wave][b];
-            if (hit) {
-              const BPair& pd = P[sc.pidx[b]];
-              lst_all[a + __popcll(bal & below)] = pd.bpos ? pd.bpos[x] : sorted_index(pd.bcand, pd.bsize, x);
-            }
-            lds_wave_sync();
-            if (lane == 0) wat[wave][b] = a + __popcll(bal);
-            lds_wave_sync();
-          }
+          lds_wave_sync();
+          if (lane == 0) wat[wave][b] = a + __popcll(bal);
+          lds_wave_sync();
         }
       }
-      __syncthreads();
     }
+    __syncthreads();
   }
 }
 
@@ -1556,11 +1558,18 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
         default: k_bcedge_count_multi<64><<<bgrid(c, sflat, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, cnt); break;
       }
     }
-    const uint32_t hub_grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((sflat + FSIG_BLOCK - 1) / FSIG_BLOCK,
-                                                                                 (uint64_t)c->num_cus * 8));
-    if (sflat)
-      k_bcedge_hub<false><<<hub_grid, FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, cnt,
+    const uint32_t hub_grid = (uint32_t)c->num_cus * 4;
+    uint64_t* hubs = nullptr;
+    unsigned int* nhubs = nullptr;
+    if (sflat && g->max_degree > kHubDeg) {
+      hubs = (uint64_t*)A.alloc(8ull * sflat);
+      nhubs = (unsigned int*)A.alloc(4);
+      if (!hubs || !nhubs) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted");
+      k_fill_u32<<<1, 64, 0, c->stream>>>(nhubs, 1, 0);
+      k_bfind_hubs<<<bgrid(c, sflat, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, g->d_off, hubs, nhubs);
+      k_bcedge_hub<false><<<hub_grid, FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, hubs, nhubs, dP, g->d_off, g->d_adj, cnt,
                                                                   nullptr, nullptr);
+    }
     FSIG_HIP(c, hipGetLastError());
     if ((rc = scan_u32_to_u64(c, cnt, gscan, pflat))) return rc;
     uint64_t total_edges;
@@ -1574,9 +1583,9 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
         default: k_bcedge_fill_multi<64><<<bgrid(c, sflat, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, gscan, lst_all); break;
       }
     }
-    if (sflat)
-      k_bcedge_hub<true><<<hub_grid, FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, nullptr,
-                                                                 gscan, lst_all);
+    if (hubs)
+      k_bcedge_hub<true><<<hub_grid, FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, hubs, nhubs, dP, g->d_off, g->d_adj,
+                                                                 nullptr, gscan, lst_all);
     k_bpair_maxlen<<<np, FSIG_BLOCK, 0, c->stream>>>(dP, gscan, dmax);
     k_bpair_base<<<(np + 255) / 256, 256, 0, c->stream>>>(gscan, dP, np, pb);
     FSIG_HIP(c, hipGetLastError());

@@ -150,6 +150,10 @@ struct PairDesc {
   uint32_t asize, bsize, bbit, flat_base;
 };
 
+// Candidates of more than kHubDeg neighbours are swept by k_cedge_hub (a
+// workgroup each) instead of one lane group.
+constexpr uint32_t kHubDeg = 2048;
+
 // bpos[C[j]] = j: the vertex -> candidate index map of one class.
 __global__ void k_scatter_pos(const uint32_t* __restrict__ cand, uint32_t n, uint32_t* __restrict__ pos) {
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) pos[cand[j]] = j;
@@ -178,6 +182,7 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_cedge_count(const PairDesc* __re
     const PairDesc& pd = P[find_pair(P, np, f)];
     const uint32_t v = pd.acand[f - pd.flat_base];
     const uint32_t s = off[v], e = off[v + 1];
+    if (e - s > kHubDeg) continue;  // k_cedge_hub
     uint32_t got = 0;
     for (uint32_t j = s; j < e; j += G) {
       const uint32_t i = j + gl;
@@ -204,13 +209,14 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_cedge_fill(const PairDesc* __res
     const PairDesc& pd = P[find_pair(P, np, f)];
     const uint32_t i0 = (uint32_t)(f - pd.flat_base);
     const uint32_t v = pd.acand[i0];
+    const uint32_t s = off[v], e = off[v + 1];
+    if (e - s > kHubDeg) continue;  // k_cedge_hub
     const uint64_t pbase = gscan[pd.flat_base];
     uint64_t at = gscan[f];
     if (gl == 0) {
       pd.off_out[i0] = (uint32_t)(at - pbase);
       if (i0 + 1 == pd.asize) pd.off_out[pd.asize] = (uint32_t)(gscan[f + 1] - pbase);
     }
-    const uint32_t s = off[v], e = off[v + 1];
     for (uint32_t j = s; j < e; j += G) {
       const uint32_t i = j + gl;
       uint32_t x = 0;
@@ -223,6 +229,82 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_cedge_fill(const PairDesc* __res
       if (hit) lst_all[at + __popcll(bal & below)] = pd.bpos ? pd.bpos[x] : sorted_index(pd.bcand, pd.bsize, x);
       at += __popcll(bal);
     }
+  }
+}
+
+// Hub candidates (degree > kHubDeg) of all pairs, gathered into one list so
+// that k_cedge_hub can deal them out one per workgroup (hubs cluster at the
+// low vertex ids of power-law graphs, i.e. at the start of every class).
+__global__ void k_find_hubs(const PairDesc* __restrict__ P, uint32_t np, uint32_t total,
+                            const uint32_t* __restrict__ off, uint32_t* __restrict__ hubs,
+                            unsigned int* __restrict__ nhubs) {
+  for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < total; f += gridDim.x * blockDim.x) {
+    const PairDesc& pd = P[find_pair(P, np, f)];
+    const uint32_t v = pd.acand[f - pd.flat_base];
+    if (off[v + 1] - off[v] > kHubDeg) hubs[atomicAdd(nhubs, 1u)] = f;
+  }
+}
+
+// Candidate edges of one hub per workgroup: the 4 wavefronts sweep one
+// contiguous quarter of its adjacency each (the fill writes each quarter
+// after the earlier ones, so lists stay ascending).
+template <bool FILL>
+__global__ void __launch_bounds__(FSIG_BLOCK) k_cedge_hub(const PairDesc* __restrict__ P, uint32_t np,
+                                                          const uint32_t* __restrict__ hubs,
+                                                          const unsigned int* __restrict__ nhubs,
+                                                          const uint32_t* __restrict__ off,
+                                                          const uint32_t* __restrict__ adj,
+                                                          const uint32_t* __restrict__ mask,
+                                                          uint32_t* __restrict__ cnt,
+                                                          const uint64_t* __restrict__ gscan,
+                                                          uint32_t* __restrict__ lst_all) {
+  constexpr uint32_t W = FSIG_BLOCK / 64;
+  __shared__ uint32_t wcnt[W];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint64_t below = (1ull << lane) - 1;
+  const uint32_t n_h = *nhubs;
+  for (uint32_t h = blockIdx.x; h < n_h; h += gridDim.x) {
+    const uint32_t fh = hubs[h];
+    const PairDesc& pd = P[find_pair(P, np, fh)];
+    const uint32_t i0 = fh - pd.flat_base;
+    const uint32_t v = pd.acand[i0];
+    const uint32_t s = off[v], len = off[v + 1] - s, q = (len + W - 1) / W;
+    const uint32_t ws = s + min(len, wave * q), we = s + min(len, (wave + 1) * q);
+    uint32_t got = 0;
+    for (uint32_t j = ws; j < we; j += 64) {
+      const uint32_t i = j + lane;
+      got += (uint32_t)__popcll(__ballot(i < we && ((mask[adj[i]] >> pd.bbit) & 1u)));
+    }
+    if (lane == 0) wcnt[wave] = got;
+    __syncthreads();
+    if (!FILL) {
+      if (threadIdx.x == 0) {
+        uint32_t sum = 0;
+        for (uint32_t w = 0; w < W; w++) sum += wcnt[w];
+        cnt[fh] = sum;
+      }
+    } else {
+      const uint64_t pbase = gscan[pd.flat_base];
+      uint64_t at = gscan[fh];
+      if (threadIdx.x == 0) {
+        pd.off_out[i0] = (uint32_t)(at - pbase);
+        if (i0 + 1 == pd.asize) pd.off_out[pd.asize] = (uint32_t)(gscan[fh + 1] - pbase);
+      }
+      for (uint32_t w = 0; w < wave; w++) at += wcnt[w];
+      for (uint32_t j = ws; j < we; j += 64) {
+        const uint32_t i = j + lane;
+        uint32_t x = 0;
+        bool hit = false;
+        if (i < we) {
+          x = adj[i];
+          hit = (mask[x] >> pd.bbit) & 1u;
+        }
+        const uint64_t bal = __ballot(hit);
+        if (hit) lst_all[at + __popcll(bal & below)] = pd.bpos ? pd.bpos[x] : sorted_index(pd.bcand, pd.bsize, x);
+        at += __popcll(bal);
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -505,6 +587,19 @@ int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_
       case 16: k_cedge_count<16><<<grid_for(c, flat, FSIG_BLOCK / 16), FSIG_BLOCK, 0, c->stream>>>(dP, np, (uint32_t)flat, g->d_off, g->d_adj, mask, cnt); break;
       default: k_cedge_count<64><<<grid_for(c, flat, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dP, np, (uint32_t)flat, g->d_off, g->d_adj, mask, cnt); break;
     }
+    uint32_t* hubs = nullptr;
+    unsigned int* nhubs = nullptr;
+    const uint32_t hub_grid = (uint32_t)c->num_cus * 4;
+    if (g->max_degree > kHubDeg) {
+      hubs = (uint32_t*)A.alloc(4ull * std::max<uint64_t>(flat, 1));
+      nhubs = (unsigned int*)A.alloc(4);
+      if (!hubs || !nhubs) return set_err(c, FSIG_ENOMEM, "cedge: arena exhausted");
+      k_fill_u32<<<1, 64, 0, c->stream>>>(nhubs, 1, 0);
+      k_find_hubs<<<grid_for(c, flat, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(dP, np, (uint32_t)flat, g->d_off, hubs,
+                                                                              nhubs);
+      k_cedge_hub<false><<<hub_grid, FSIG_BLOCK, 0, c->stream>>>(dP, np, hubs, nhubs, g->d_off, g->d_adj, mask, cnt,
+                                                               nullptr, nullptr);
+    }
     FSIG_HIP(c, hipGetLastError());
     if ((rc = scan_u32_to_u64(c, cnt, gscan, flat))) return rc;
     uint64_t total_edges;
@@ -516,6 +611,9 @@ int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_
       case 16: k_cedge_fill<16><<<grid_for(c, flat, FSIG_BLOCK / 16), FSIG_BLOCK, 0, c->stream>>>(dP, np, (uint32_t)flat, g->d_off, g->d_adj, mask, gscan, lst_all); break;
       default: k_cedge_fill<64><<<grid_for(c, flat, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dP, np, (uint32_t)flat, g->d_off, g->d_adj, mask, gscan, lst_all); break;
     }
+    if (hubs)
+      k_cedge_hub<true><<<hub_grid, FSIG_BLOCK, 0, c->stream>>>(dP, np, hubs, nhubs, g->d_off, g->d_adj, mask, nullptr,
+                                                              gscan, lst_all);
     k_gather_u64<<<(np + 255) / 256, 256, 0, c->stream>>>(gscan, dP, np, pbase);
     FSIG_HIP(c, hipGetLastError());
     std::vector<uint64_t> hb(np);

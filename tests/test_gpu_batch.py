@@ -99,6 +99,11 @@ def test_batch_hub_candidates(gpu_ctx):
     assert g.max_degree > 2048
     dg = gpu_ctx.upload(g)
     qs, ref = zip(*_bounded(dg, [q for _, q in W.walk_queries(g, 24, 3, 6, 23)], 20_000_000))
+    off, adj, lab = g.arrays()
+    for q, c in list(zip(qs, ref))[:6]:  # the plain path's hub lists against the oracle
+        want, _ = O.match(q.labels, q.adjm(), off, adj, lab, threads=8, limit=10 ** 8)
+        if want != O.TOO_MANY:
+            assert c == want
     for strategy in (3, 1):  # lockstep and persistent joins over the same lists
         try:
             gpu_ctx.set_tune(lanes_per_row=16 if strategy == 1 else 0, strategy=strategy)
