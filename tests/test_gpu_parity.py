@@ -249,3 +249,26 @@ def test_lanes_per_row_do_not_change_results(gpu_ctx, lanes):
             assert F.match(dg, q, alg, F.MODE_COUNT).count == want
     finally:
         gpu_ctx.set_tune()
+
+
+def test_hub_candidates_unlabeled_closed_forms(gpu_ctx):
+    # R-MAT with hubs above the 2048-neighbour threshold: candidate edges of
+    # the hubs come from the per-hub workgroup sweep.  Monomorphism counts
+    # against closed forms: triangle tr(A^3), 4-cycle tr(A^4) - 2 sum d^2 + 2m
+    import scipy.sparse as sp
+    g = F.HostGraph.rmat(12, 150000, seed=27)
+    assert g.max_degree > 2048
+    off, adj, _ = g.arrays()
+    A = sp.csr_matrix((np.ones(len(adj)), adj.astype(np.int64), off.astype(np.int64)), shape=(g.n, g.n))
+    A2 = A @ A
+    d = np.diff(off).astype(np.float64)
+    want = {"triangle": int(round(A2.multiply(A).sum())),
+            "cycle4": int(round(A2.multiply(A2).sum() - 2 * (d * d).sum() + d.sum()))}
+    dg = gpu_ctx.upload(g)
+    qs = {"triangle": F.Query(3, [0, 0, 0], [(0, 1), (1, 2), (0, 2)]),
+          "cycle4": F.Query(4, [0] * 4, [(0, 1), (1, 2), (2, 3), (3, 0)])}
+    for name, q in qs.items():
+        assert F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT).count == want[name], name
+    counts, _ = F.match_batch(dg, list(qs.values()))
+    assert counts.tolist() == [want["triangle"], want["cycle4"]]
+    dg.free()
