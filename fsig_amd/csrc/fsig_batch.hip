@@ -241,15 +241,49 @@ __global__ void k_bpair_base(const uint64_t* __restrict__ gscan, const BPair* __
 
 // One (query, class a) with every target class b of a pair a -> b: the
 // candidate-edge lists of all of a's pairs come from one sweep over each
-// candidate's adjacency (one mask gather per neighbour serves every target).
+// candidate's adjacency.
 struct BSrc {
   const uint32_t* acand;
-  const uint32_t* mask;
+  const uint32_t* mask;        // the query's class mask (null: bitmaps)
   uint32_t asize;
   uint32_t tmask;
   uint64_t flat;
-  uint32_t pidx[MAXQ];  // pair index of target b
+  uint32_t pidx[MAXQ];         // pair index of target b
+  const uint32_t* bits[MAXQ];  // candidate bitmap (n bits) of target class b
 };
+
+// Target classes among tm that vertex w is a candidate of: one gather of
+// the query's class-mask word, or (graphs of at most 4M vertices) one bit
+// test per target in n-bit class bitmaps, which stay in L2 there while the
+// 4-byte-per-vertex masks of a whole batch do not.
+__device__ __forceinline__ uint32_t target_bits(const BSrc& sc, uint32_t w, uint32_t tm) {
+  if (sc.mask) return sc.mask[w] & tm;
+  uint32_t m = 0;
+  for (uint32_t t = tm; t; t &= t - 1) {
+    const uint32_t b = __builtin_ctz(t);
+    m |= ((sc.bits[b][w >> 5] >> (w & 31u)) & 1u) << b;
+  }
+  return m;
+}
+
+// Candidate bitmaps of the target classes: class j's candidates set bits
+// in its n-bit map (the maps are zeroed before).
+struct BBits {
+  const uint32_t* cand;
+  uint32_t* bits;
+  uint32_t size;
+  uint32_t pad;
+  uint64_t flat;
+};
+
+__global__ void k_bset_bits(const BBits* __restrict__ T, uint32_t nt, uint64_t total) {
+  for (uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; f < total;
+       f += (uint64_t)gridDim.x * blockDim.x) {
+    const BBits& t = T[find_flat(T, nt, f)];
+    const uint32_t v = t.cand[f - t.flat];
+    atomicOr(&t.bits[v >> 5], 1u << (v & 31u));
+  }
+}
 
 // Candidates of more than kHubDeg neighbours are swept by k_bcedge_hub (a
 // workgroup per candidate) instead of one lane group.
@@ -277,7 +311,7 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_count_multi(const BSrc* _
       for (uint32_t t = tm; t; t &= t - 1) acc[gw][__builtin_ctz(t)] = 0;
     for (uint32_t j = s; j < e; j += G) {
       const uint32_t i = j + gl;
-      const uint32_t m = i < e ? (sc.mask[adj[i]] & tm) : 0u;
+      const uint32_t m = i < e ? target_bits(sc, adj[i], tm) : 0u;
       for (uint32_t t = tm; t; t &= t - 1) {
         const uint32_t b = __builtin_ctz(t);
         const uint32_t got = (uint32_t)__popcll(__ballot((m >> b) & 1u) & gmask);
@@ -330,7 +364,7 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_fill_multi(const BSrc* __
       uint32_t x = 0, m = 0;
       if (i < e) {
         x = adj[i];
-        m = sc.mask[x] & tm;
+        m = target_bits(sc, x, tm);
       }
       for (uint32_t t = tm; t; t &= t - 1) {
         const uint32_t b = __builtin_ctz(t);
@@ -402,7 +436,7 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_hub(const BSrc* __restric
     lds_wave_sync();
     for (uint32_t j = ws; j < we; j += 64) {
       const uint32_t i = j + lane;
-      const uint32_t m = i < we ? (sc.mask[adj[i]] & tm) : 0u;
+      const uint32_t m = i < we ? target_bits(sc, adj[i], tm) : 0u;
       for (uint32_t t = tm; t; t &= t - 1) {
         const uint32_t b = __builtin_ctz(t);
         const uint32_t got = (uint32_t)__popcll(__ballot((m >> b) & 1u));
@@ -439,7 +473,7 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_hub(const BSrc* __restric
         uint32_t x = 0, m = 0;
         if (i < we) {
           x = adj[i];
-          m = sc.mask[x] & tm;
+          m = target_bits(sc, x, tm);
         }
         for (uint32_t t = tm; t; t &= t - 1) {
           const uint32_t b = __builtin_ctz(t);
@@ -1519,6 +1553,38 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
         pd.bpos = it->second;
       }
     }
+    // candidate bitmaps of every target class, n bits each (graphs of at
+    // most 4M vertices)
+    const uint64_t nwords = (n + 31ull) / 32;
+    const bool use_bits = n <= (1u << 22);
+    std::vector<uint32_t*> tbits(std::max<uint32_t>(ncls, 1), nullptr);
+    if (use_bits) {
+      std::vector<BBits> bt;
+      uint64_t bflat = 0;
+      std::vector<uint32_t> tgt;
+      for (uint32_t i = 0; i < nq; i++)
+        for (uint32_t x = 0; x < MAXQ * MAXQ && active[i]; x++)
+          if (pair_of[i][x] >= 0) {
+            const uint32_t gb = cls_base[i] + x % MAXQ;
+            if (!tbits[gb]) { tbits[gb] = reinterpret_cast<uint32_t*>(1); tgt.push_back(gb); }
+          }
+      uint32_t* base = (uint32_t*)A.alloc(4ull * nwords * std::max<size_t>(tgt.size(), 1));
+      if (!base) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted for candidate bitmaps");
+      for (size_t j = 0; j < tgt.size(); j++) {
+        tbits[tgt[j]] = base + j * nwords;
+        if (size[tgt[j]]) {
+          bt.push_back(BBits{cand[tgt[j]], tbits[tgt[j]], size[tgt[j]], 0, bflat});
+          bflat += size[tgt[j]];
+        }
+      }
+      if (!tgt.empty())
+        k_fill_u32<<<bgrid(c, nwords * tgt.size(), FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(base, nwords * tgt.size(), 0);
+      const BBits* dbt = upload_table(c, bt);
+      if (!dbt) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted");
+      if (bflat)
+        k_bset_bits<<<bgrid(c, bflat, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(dbt, (uint32_t)bt.size(), bflat);
+      FSIG_HIP(c, hipGetLastError());
+    }
     // sources: every (query, class a) with its targets, one sweep each
     std::vector<BSrc> srcs;
     uint64_t sflat = 0;
@@ -1532,11 +1598,12 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
           if (pi < 0) continue;
           sc.tmask |= 1u << b;
           sc.pidx[b] = (uint32_t)pi;
+          sc.bits[b] = tbits[cls_base[i] + b];
         }
         if (!sc.tmask) continue;
         const uint32_t ga = cls_base[i] + a;
         sc.acand = cand[ga];
-        sc.mask = masks + (uint64_t)i * n;
+        sc.mask = use_bits ? nullptr : masks + (uint64_t)i * n;
         sc.asize = size[ga];
         sc.flat = sflat;
         sflat += size[ga];
