@@ -336,13 +336,19 @@ extern "C" int fsig_server_wait(fsig_server* s, uint64_t ticket, int64_t timeout
                                 double* latency_ms) {
   if (!s) return FSIG_EINVAL;
   std::unique_lock<std::mutex> lk(s->mu);
-  auto it = s->results.find(ticket);
-  if (it == s->results.end()) return FSIG_ENOTFOUND;
-  auto ready = [&] { return s->results[ticket].rc != FSIG_EAGAIN; };
+  if (s->results.find(ticket) == s->results.end()) return FSIG_ENOTFOUND;
+  // ready when finished, or gone (another waiter on the same ticket took it)
+  auto ready = [&] {
+    auto it = s->results.find(ticket);
+    return it == s->results.end() || it->second.rc != FSIG_EAGAIN;
+  };
   if (timeout_ms < 0) s->cv_done.wait(lk, ready);
   else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) return FSIG_EAGAIN;
-  const Result r = s->results[ticket];
-  s->results.erase(ticket);
+  auto it = s->results.find(ticket);
+  if (it == s->results.end()) return FSIG_ENOTFOUND;
+  const Result r = it->second;
+  s->results.erase(it);
+  s->cv_done.notify_all();
   if (count) *count = r.count;
   if (latency_ms) *latency_ms = r.latency_ms;
   return r.rc;

@@ -221,7 +221,8 @@ int fsig_server_submit(fsig_server* s, uint32_t k, const int32_t* qlabels, const
 /* Waits up to timeout_ms (< 0: without limit) for a ticket; returns the
  * query's status (FSIG_EAGAIN: not finished, the ticket stays valid) and on
  * completion its count and submit-to-completion latency.  A completed
- * ticket is released by the call that returns it. */
+ * ticket is released by the call that returns it; any other call waiting on
+ * it then returns FSIG_ENOTFOUND. */
 int fsig_server_wait(fsig_server* s, uint64_t ticket, int64_t timeout_ms, uint64_t* count, double* latency_ms);
 int fsig_server_get_stats(fsig_server* s, fsig_server_stats* out);
 /* Projected bytes the server admits the query with. */
