@@ -172,3 +172,22 @@ def test_large_file_parsed_in_slices(tmp_path):
     with pytest.raises(F.FsigError) as e:
         F.HostGraph.read(p)
     assert e.value.code == -3  # truncated or malformed record
+
+
+def test_walk_bias_direction():
+    # bias > 0 steps towards high-degree neighbours, bias < 0 towards low
+    g = F.HostGraph.chung_lu(20000, 100000, 2.2, 61)
+    off, _, _ = g.arrays()
+    deg = np.diff(off.astype(np.int64))
+    mean = {}
+    for bias in (-4, 0, 4):
+        ds = []
+        for s in range(40):
+            q = g.walk_query(6, 5, 15, seed=1000 + s, bias=bias, max_attempts=2000)
+            ds.extend(deg[q.walk].tolist())
+        mean[bias] = float(np.mean(ds))
+    assert mean[-4] < mean[0] < mean[4]
+    # the same seed and bias give the same query (starts are cached per graph)
+    a = g.walk_query(6, 5, 15, seed=7, bias=-2)
+    b = g.walk_query(6, 5, 15, seed=7, bias=-2)
+    assert a.edges.tolist() == b.edges.tolist() and list(a.labels) == list(b.labels)
