@@ -50,6 +50,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    """The host CPU's model name, from /proc/cpuinfo (None where unreadable)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(g, queries, seconds_target=20.0):
     """Oracle on the host: the dense query is counted in full (it also checks
     the GPU count at full size); the sparse query is sampled: the search
@@ -84,6 +96,7 @@ def cpu_baseline(g, queries, seconds_target=20.0):
             if not stopped:
                 checks[name] = cnt
     return {"value": total_emb / total_s, "unit": "embeddings/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
             "sample": "oracle backtracking (oracle/oracle.c), roots handed out dynamically to "
                       f"{threads} threads, {total_s:.1f} s: " + "; ".join(parts)}, checks
 
