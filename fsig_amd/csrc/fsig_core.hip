@@ -316,6 +316,9 @@ int fsig_graph_upload_pcsr(fsig_ctx* c, fsig_graph* g, uint32_t num_units, const
       if (unit_edges[u][i] >= g->n) return set_err(c, FSIG_EINVAL, "upload_pcsr: neighbour id out of range");
   }
   FSIG_HIP(c, hipSetDevice(c->device));
+  // a failed re-upload leaves the graph without a PCSR, never with freed buffers
+  g->has_pcsr = false;
+  g->num_units = 0;
   if (g->d_groups) { (void)hipFree(g->d_groups); g->d_groups = nullptr; }
   if (g->d_pedges) { (void)hipFree(g->d_pedges); g->d_pedges = nullptr; }
   if (g->d_units) { (void)hipFree(g->d_units); g->d_units = nullptr; }
