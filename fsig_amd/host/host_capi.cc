@@ -55,6 +55,15 @@ struct BinFile {
   template <class T>
   bool get(T* p, size_t n) { return n == 0 || std::fread(p, sizeof(T), n, f) == n; }
   bool at_end() { return std::fgetc(f) == EOF; }
+  // true when at least `bytes` remain after the read position: a corrupt
+  // header is rejected before it sizes a buffer
+  bool has(uint64_t bytes) {
+    const long at = std::ftell(f);
+    if (at < 0 || std::fseek(f, 0, SEEK_END) != 0) return false;
+    const long end = std::ftell(f);
+    if (end < 0 || std::fseek(f, at, SEEK_SET) != 0) return false;
+    return (uint64_t)(end - at) >= bytes;
+  }
 };
 
 }  // namespace
@@ -235,6 +244,7 @@ fsig_hgraph* fsig_hg_load_bin(const char* path, int* status) {
   auto g = std::make_unique<fsig_hgraph>();
   uint32_t m = 0;
   if (!b.get(&g->n, 1) || !b.get(&m, 1) || !b.get(&g->num_labels, 1)) return fail(kIoBadHeader);
+  if (!b.has(4ull * g->n + 4ull * (g->n + 1ull) + 8ull * m)) return fail(kIoBadHeader);
   g->labels.resize(g->n);
   g->off.resize(g->n + 1ull);
   if (!b.get(g->labels.data(), g->n) || !b.get(g->off.data(), g->n + 1ull)) return fail(kIoBadRecord);
@@ -272,12 +282,15 @@ fsig_hpcsr* fsig_pcsr_load_bin(const char* path, int* status) {
   auto p = std::make_unique<fsig_hpcsr>();
   uint32_t nu = 0;
   if (!b.get(&p->n, 1) || !b.get(&p->num_edges, 1) || !b.get(&p->num_labels, 1)) return fail(kIoBadHeader);
+  if (!b.has(4ull * p->n + 4)) return fail(kIoBadHeader);
   p->labels.resize(p->n);
   if (!b.get(p->labels.data(), p->n) || !b.get(&nu, 1)) return fail(kIoBadRecord);
   if (p->num_labels < 1 || nu != (uint64_t)p->num_labels * (p->num_labels + 1) / 2) return fail(kIoBadHeader);
+  if (!b.has(8ull * nu)) return fail(kIoBadHeader);
   p->units.resize(nu);
   for (PCSRUnit& u : p->units) {
     if (!b.get(&u.num_keys_, 1) || !b.get(&u.num_edges_, 1)) return fail(kIoBadRecord);
+    if (!b.has(128ull * u.num_keys_ + 4ull * u.num_edges_)) return fail(kIoBadRecord);
     u.offset_.resize(32ull * u.num_keys_);
     u.edges_.resize(u.num_edges_);
     if (!b.get(u.offset_.data(), u.offset_.size()) || !b.get(u.edges_.data(), u.edges_.size()))

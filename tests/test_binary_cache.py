@@ -78,3 +78,30 @@ def test_damaged_caches_are_rejected(tmp_path):
     open(pp, "ab").write(b"\1\0\0\0")
     with pytest.raises(F.FsigError):
         F.HostPCSR.load_bin(pp)
+
+
+def test_huge_counts_in_a_header_are_rejected_before_allocation(tmp_path):
+    g = _graph()
+    p = str(tmp_path / "g.csr.bin")
+    g.save_bin(p)
+    raw = bytearray(open(p, "rb").read())
+    for field in (0, 4):                        # vertex count, edge count
+        bad = bytearray(raw)
+        bad[field:field + 4] = struct.pack("<I", 0xFFFFFFF0)
+        q = str(tmp_path / "huge.bin")
+        open(q, "wb").write(bytes(bad))
+        with pytest.raises(F.FsigError):
+            F.HostGraph.load_bin(q)
+    pc = g.pcsr()
+    pp = str(tmp_path / "g.pcsr.bin")
+    pc.save_bin(pp, g)
+    raw = bytearray(open(pp, "rb").read())
+    n = struct.unpack_from("<I", raw, 0)[0]
+    first_unit = 12 + 4 * n + 4                  # header, labels, unit count
+    for off in (first_unit, first_unit + 4):    # key count, edge count of unit 0
+        bad = bytearray(raw)
+        bad[off:off + 4] = struct.pack("<I", 0x7FFFFFFF)
+        q = str(tmp_path / "huge.pcsr.bin")
+        open(q, "wb").write(bytes(bad))
+        with pytest.raises(F.FsigError):
+            F.HostPCSR.load_bin(q)
