@@ -141,6 +141,13 @@ int read_plain_graph(const std::string& path, bool is_labeled, UndirectedPlainGr
   }
   for (auto& sl : slices)
     if (sl.rc) return sl.rc;
+  // every id up to the largest needs a v line, so an id at or past the number
+  // of v lines has a gap below it: reject it before sizing the vertex arrays
+  uint64_t vlines = 0;
+  for (auto& sl : slices) vlines += sl.vertices.size();
+  for (auto& sl : slices)
+    for (auto& v : sl.vertices)
+      if (v.first >= vlines) return kIoMissingVertex;
   for (auto& sl : slices)
     for (auto& v : sl.vertices) g.add_vertex(v.first, v.second);
   for (Vertex v = 0; v < g.num_vertices_; v++)

@@ -127,6 +127,13 @@ def test_reader_errors():
         with pytest.raises(F.FsigError) as e:  # endpoint without v line
             F.HostGraph.read(_write(td, "t 2 1\nv 0 0 1\nv 1 0 1\ne 0 5"))
         assert e.value.code == -5
+        with pytest.raises(F.FsigError) as e:  # a huge id: rejected before any array is sized from it
+            F.HostGraph.read(_write(td, "t 2 1\nv 0 0 1\nv 4000000000 0 1\ne 0 4000000000"))
+        assert e.value.code == -4
+        # a duplicated v line does not hide a gap
+        with pytest.raises(F.FsigError) as e:
+            F.HostGraph.read(_write(td, "t 3 1\nv 0 0 1\nv 0 0 1\nv 2 0 1\ne 0 2"))
+        assert e.value.code == -4
 
 
 def test_generators_are_deterministic_and_sized():
