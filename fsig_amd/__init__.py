@@ -276,7 +276,10 @@ class HostGraph:
         return out
 
     def pcsr(self, reference_unit_ids: bool = False) -> "HostPCSR":
-        return HostPCSR(self._L.fsig_pcsr_build(self._h, int(reference_unit_ids)), reference_unit_ids)
+        h = self._L.fsig_pcsr_build(self._h, int(reference_unit_ids))
+        if not h:
+            raise FsigError(-1, f"PCSR is built for at most 4096 labels (graph has {self.num_labels})")
+        return HostPCSR(h, reference_unit_ids)
 
     def walk_query(self, k: int, min_edges: int = 0, max_edges: int = 10 ** 6, seed: int = 1, bias: int = 0,
                    max_attempts: int = 100000) -> "Query":

@@ -207,6 +207,7 @@ int fsig_hg_walk_query(const fsig_hgraph* g, uint32_t k, uint32_t min_e, uint32_
 }
 
 fsig_hpcsr* fsig_pcsr_build(const fsig_hgraph* g, int reference_unit_ids) {
+  if (!g || g->num_labels > kPcsrMaxLabels) return nullptr;
   auto* p = new fsig_hpcsr();
   pcsr_from_csr(g->n, g->off.data(), g->adj.data(), g->labels.data(), g->num_labels, reference_unit_ids != 0,
                 p->units);

@@ -192,7 +192,7 @@ void pcsr_from_csr(uint32_t n, const uint32_t* off, const uint32_t* adj, const i
 
 int UndirectedPlainToPCSRTransformer::transform(const Graph* input, Graph*& output) {
   const auto* g = dynamic_cast<const UndirectedPlainGraph*>(input);
-  if (!g) return -1;
+  if (!g || g->num_labels_ > kPcsrMaxLabels) return -1;
   // Same adjacency the reference uses: each list sorted ascending
   // (graph_format_transformer.cc:79-83), duplicates kept.
   const Vertex n = g->num_vertices_;

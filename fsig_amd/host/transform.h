@@ -60,6 +60,10 @@ class UndirectedPlainToPCSRTransformer : public GraphFormatTransformer {
 void csr_from_edge_list(uint32_t n, const std::vector<std::pair<uint32_t, uint32_t>>& edges,
                         std::vector<uint32_t>& offset, std::vector<uint32_t>& adj, int threads = 0);
 
+// Largest label count a PCSR is built for: one unit per label pair, so 4096
+// labels already make 8.4M units.
+constexpr int32_t kPcsrMaxLabels = 4096;
+
 // PCSR built directly from CSR arrays (same layout as the transformer).
 void pcsr_from_csr(uint32_t n, const uint32_t* offset, const uint32_t* adj, const int32_t* labels,
                    int32_t num_labels, bool reference_unit_ids, std::vector<PCSRUnit>& units);

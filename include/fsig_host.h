@@ -48,7 +48,8 @@ void fsig_hg_free(fsig_hgraph* g);
 int fsig_hg_walk_query(const fsig_hgraph* g, uint32_t k, uint32_t min_e, uint32_t max_e, uint64_t seed,
                        int bias, uint32_t max_attempts, int32_t* labels, uint32_t* edges, uint32_t* walk);
 
-/* PCSR of a host graph (graph_format_transformer.cc:39-215). */
+/* PCSR of a host graph (graph_format_transformer.cc:39-215).  NULL for a
+ * graph with more than 4096 labels (one unit per label pair). */
 typedef struct fsig_hpcsr fsig_hpcsr;
 fsig_hpcsr* fsig_pcsr_build(const fsig_hgraph* g, int reference_unit_ids);
 uint32_t fsig_pcsr_num_units(const fsig_hpcsr* p);

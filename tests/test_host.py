@@ -198,3 +198,10 @@ def test_walk_bias_direction():
     a = g.walk_query(6, 5, 15, seed=7, bias=-2)
     b = g.walk_query(6, 5, 15, seed=7, bias=-2)
     assert a.edges.tolist() == b.edges.tolist() and list(a.labels) == list(b.labels)
+
+
+def test_pcsr_label_cap():
+    g = F.HostGraph.chung_lu(6000, 20000, 2.3, 3).set_labels("uniform", 5000, seed=2)
+    with pytest.raises(F.FsigError):
+        g.pcsr()
+    assert F.HostGraph.chung_lu(600, 2000, 2.3, 3).set_labels("uniform", 64, seed=2).pcsr().num_units == 64 * 65 // 2
