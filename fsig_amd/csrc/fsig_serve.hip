@@ -26,6 +26,10 @@
 
 namespace {
 
+// Largest label count the served loop and fsig_graph_label_pairs take: the
+// label-pair table holds L * L u64 counts (128 MiB at 4096).
+constexpr uint32_t kServeMaxLabels = 4096;
+
 // Directed label-pair edge counts: pairs[a * L + b] = number of adjacency
 // entries (v, w) with label(v) = a and label(w) = b.  One wavefront per
 // vertex; per-block LDS histogram when L <= 64.
@@ -127,6 +131,7 @@ extern "C" int fsig_project_bytes(uint32_t n, uint32_t num_labels, const uint64_
 extern "C" int fsig_graph_label_pairs(fsig_ctx* c, const fsig_graph* g, uint64_t* pair_edges) {
   if (!c) return FSIG_EINVAL;
   if (!g || !pair_edges) return set_err(c, FSIG_EINVAL, "label pairs: null argument");
+  if (g->num_labels > kServeMaxLabels) return set_err(c, FSIG_EINVAL, "label pairs: more than 4096 labels");
   FSIG_HIP(c, hipSetDevice(c->device));
   const uint32_t L = g->num_labels;
   const size_t mk = c->arena.mark();
@@ -257,6 +262,8 @@ extern "C" int fsig_server_create(int device, const fsig_graph* g, uint32_t work
                                   uint32_t max_batch, const fsig_tune* tune, fsig_server** out) {
   if (!out || !g || workers == 0 || workers > 16 || max_batch == 0) return FSIG_EINVAL;
   *out = nullptr;
+  // admission keeps an L x L table of label-pair adjacency counts on the host
+  if (g->num_labels > kServeMaxLabels) return FSIG_EINVAL;
   if (hipSetDevice(device) != hipSuccess) return FSIG_EHIP;
   size_t freeb = 0, totalb = 0;
   if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return FSIG_EHIP;

@@ -203,6 +203,8 @@ int fsig_stream_copy(fsig_ctx* ctx, uint64_t bytes, uint32_t iters, double* gbps
  * overlap on the device.  A batch that still runs out of arena falls back
  * to fsig_match per query, whose join is memory-blocked.  tune (NULL:
  * lockstep join with lanes per query, no autotuning) is set on every worker.
+ * The graph may have at most 4096 labels (admission keeps an L x L table of
+ * label-pair adjacency counts; fsig_graph_label_pairs has the same limit).
  * Replaces the reference's one query per process with a static device
  * budget (main/run.cc:101-140, memory_manager/memory_manager.h:220-242). */
 typedef struct fsig_server_stats {
