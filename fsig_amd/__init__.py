@@ -588,6 +588,7 @@ class Server:
     def __init__(self, dg: "DeviceGraph", workers: int = 4, hbm_budget: int = 0, max_batch: int = 64,
                  tune: Tune = None):
         self._L = hip_lib()
+        self._h = None
         self.dg = dg  # keeps the graph alive
         h = _vp()
         rc = self._L.fsig_server_create(dg.ctx.device, dg._h, workers, hbm_budget, max_batch,

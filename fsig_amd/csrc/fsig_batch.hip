@@ -1291,6 +1291,8 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
                                 const uint32_t* const* qadj, uint64_t* counts_out, double* join_ms) {
   if (!c) return FSIG_EINVAL;
   if (!g || !ks || !qlabels || !qoff || !qadj || !counts_out) return set_err(c, FSIG_EINVAL, "batch: null argument");
+  if (join_ms) *join_ms = 0;
+  if (nq == 0) return FSIG_OK;
   FSIG_HIP(c, hipSetDevice(c->device));
   const uint32_t n = g->n;
   Arena& A = c->arena;

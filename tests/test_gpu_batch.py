@@ -77,6 +77,10 @@ def test_batch_autotuner_and_repeat(gpu_ctx):
     # lanes 0: the lockstep join with each query's own lane count
     assert lanes in (0, 4, 8, 16, 32, 64) and engine in ("lockstep", "persistent")
     assert a.tolist() == list(ref)
+    # an empty batch returns at once and keeps the tuning decision
+    e, ms = F.match_batch(dg, [])
+    assert len(e) == 0 and ms == 0.0
+    assert F.last_batch_tune(gpu_ctx) == (lanes, tile, engine)
 
 
 def test_batch_lanes_per_query(gpu_ctx):

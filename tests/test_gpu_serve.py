@@ -104,3 +104,17 @@ def test_server_unknown_ticket_and_projection(gpu_ctx):
         t = srv.submit(q)
         assert srv.wait(t, 60_000)[0] == F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT).count
     dg.free()
+
+
+def test_server_refuses_more_than_4096_labels(gpu_ctx):
+    g = F.HostGraph.chung_lu(12000, 40000, 2.3, 8).set_labels("uniform", 5000, seed=9)
+    dg = gpu_ctx.upload(g)
+    with pytest.raises(F.FsigError) as e:
+        F.Server(dg, workers=1, hbm_budget=1 << 28, max_batch=4)
+    assert e.value.code == F.FSIG_EINVAL
+    with pytest.raises(F.FsigError):
+        F.label_pairs(dg)
+    # the graph itself still matches
+    q = g.walk_query(3, 2, 3, seed=4)
+    assert F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT).count >= 1
+    dg.free()
