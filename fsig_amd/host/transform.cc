@@ -2,8 +2,8 @@
 #include "transform.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
-#include <deque>
 #include <thread>
 
 namespace fsig {
@@ -121,72 +121,117 @@ void pcsr_from_csr(uint32_t n, const uint32_t* off, const uint32_t* adj, const i
     return (uint32_t)(ref_ids ? UndirectedPCSRGraph::reference_edge_label(a, b)
                               : UndirectedPCSRGraph::injective_edge_label(a, b));
   };
-  // Pass 1: per unit, keys in ascending vertex order and their list lengths.
-  std::vector<std::vector<uint32_t>> keys(num_units), key_len(num_units);
+  // Pass 1, one sweep of the adjacency: per unit its keys in ascending vertex
+  // order, each key's list length, and the lists themselves back to back in
+  // key order (a key's list keeps its adjacency order, ascending).
+  std::vector<std::vector<uint32_t>> keys(num_units), key_len(num_units), lists(num_units);
   std::vector<uint32_t> last_key(num_units, kInvalid);
   for (uint32_t v = 0; v < n; v++) {
     for (uint32_t i = off[v]; i < off[v + 1]; i++) {
-      uint32_t u = unit_of(labels[v], labels[adj[i]]);
+      const uint32_t u = unit_of(labels[v], labels[adj[i]]);
       if (last_key[u] != v) {
         last_key[u] = v;
         keys[u].push_back(v);
         key_len[u].push_back(0);
       }
       key_len[u].back()++;
+      lists[u].push_back(adj[i]);
     }
   }
   units.assign(num_units, PCSRUnit());
-  std::vector<uint32_t> key_index;  // position of a key in keys[u]
-  for (uint32_t u = 0; u < num_units; u++) {
-    PCSRUnit& unit = units[u];
-    const uint32_t K = (uint32_t)keys[u].size();
-    unit.num_keys_ = K;
-    uint64_t ne = 0;
-    for (uint32_t l : key_len[u]) ne += l;
-    unit.num_edges_ = (uint32_t)ne;
-    unit.offset_.assign((size_t)32 * K, 0);
-    for (size_t w = 0; w < (size_t)32 * K; w += 2) unit.offset_[w] = kInvalid;
-    unit.edges_.assign(ne, kInvalid);
-    if (K == 0) continue;
-    // buckets by hash, keys in insertion order
-    std::vector<std::vector<uint32_t>> bucket(K);
-    for (uint32_t j = 0; j < K; j++) {
-      uint32_t key = keys[u][j];
-      bucket[murmur_hash2(&key, 4, kHashSeed) % K].push_back(j);
-    }
-    std::deque<uint32_t> empty;
-    for (uint32_t b = 0; b < K; b++)
-      if (bucket[b].empty()) empty.push_back(b);
-    for (uint32_t b = 0; b < K; b++) {
-      const auto& bk = bucket[b];
-      if (bk.empty()) continue;
-      uint32_t group = b;
-      size_t placed = 0;
-      const size_t full = bk.size() / 15;
-      for (size_t f = 0; f < full; f++) {
-        for (uint32_t s = 0; s < 15; s++) unit.offset_[(size_t)32 * group + 2 * s] = keys[u][bk[placed++]];
-        uint32_t next = empty.front();
-        empty.pop_front();
-        unit.offset_[(size_t)32 * group + 30] = next;
-        group = next;
+  // Pass 2, units in parallel (largest first): hash the keys into K groups of
+  // 15 slots (counting sort by bucket, insertion order kept within a bucket),
+  // chain a full bucket's overflow through the empty groups in ascending
+  // order, then copy each key's list into place group by group, slot by slot.
+  std::vector<uint32_t> order(num_units);
+  for (uint32_t u = 0; u < num_units; u++) order[u] = u;
+  std::sort(order.begin(), order.end(),
+            [&](uint32_t a, uint32_t b) { return lists[a].size() > lists[b].size(); });
+  std::atomic<uint32_t> next{0};
+  auto worker = [&]() {
+    std::vector<uint32_t> hash, bstart, byb, slot_key, kstart;
+    for (uint32_t t = next++; t < num_units; t = next++) {
+      const uint32_t u = order[t];
+      PCSRUnit& unit = units[u];
+      const uint32_t K = (uint32_t)keys[u].size();
+      const uint64_t ne = lists[u].size();
+      unit.num_keys_ = K;
+      unit.num_edges_ = (uint32_t)ne;
+      unit.offset_.assign((size_t)32 * K, 0);
+      for (size_t w = 0; w < (size_t)32 * K; w += 2) unit.offset_[w] = kInvalid;
+      unit.edges_.resize(ne);
+      if (K == 0) continue;
+      const uint32_t* ku = keys[u].data();
+      hash.resize(K);
+      bstart.assign((size_t)K + 1, 0);
+      for (uint32_t j = 0; j < K; j++) {
+        uint32_t key = ku[j];
+        hash[j] = murmur_hash2(&key, 4, kHashSeed) % K;
+        bstart[hash[j] + 1]++;
       }
-      for (uint32_t s = 0; placed < bk.size(); s++)
-        unit.offset_[(size_t)32 * group + 2 * s] = keys[u][bk[placed++]];
-    }
-    // key -> index map for list lookup (replaces the linear find)
-    key_index.assign(0, 0);
-    // Lists are written group by group, slot by slot.
-    uint32_t pos = 0;
-    for (uint32_t grp = 0; grp < K; grp++) {
-      for (uint32_t s = 0; s < 15; s++) {
-        uint32_t key = unit.offset_[(size_t)32 * grp + 2 * s];
-        if (key == kInvalid) break;
-        unit.offset_[(size_t)32 * grp + 2 * s + 1] = pos;
-        for (uint32_t i = off[key]; i < off[key + 1]; i++)
-          if (unit_of(labels[key], labels[adj[i]]) == u) unit.edges_[pos++] = adj[i];
-        unit.offset_[(size_t)32 * grp + 2 * (s + 1) + 1] = pos;
+      for (uint32_t b = 0; b < K; b++) bstart[b + 1] += bstart[b];
+      byb.resize(K);
+      {
+        std::vector<uint32_t>& cur = kstart;  // reused as the fill cursor
+        cur.assign(bstart.begin(), bstart.end() - 1);
+        for (uint32_t j = 0; j < K; j++) byb[cur[hash[j]]++] = j;
       }
+      // key index held by each (group, slot); kInvalid where empty
+      slot_key.assign((size_t)15 * K, kInvalid);
+      uint32_t empty_at = 0;  // FIFO over the empty buckets in ascending order
+      auto next_empty = [&]() {
+        while (bstart[empty_at + 1] != bstart[empty_at]) empty_at++;
+        return empty_at++;
+      };
+      for (uint32_t b = 0; b < K; b++) {
+        const uint32_t* bk = byb.data() + bstart[b];
+        const uint32_t cnt = bstart[b + 1] - bstart[b];
+        if (cnt == 0) continue;
+        uint32_t group = b, placed = 0;
+        for (uint32_t f = 0; f < cnt / 15; f++) {
+          for (uint32_t s = 0; s < 15; s++) {
+            slot_key[(size_t)15 * group + s] = bk[placed];
+            unit.offset_[(size_t)32 * group + 2 * s] = ku[bk[placed++]];
+          }
+          const uint32_t nx = next_empty();
+          unit.offset_[(size_t)32 * group + 30] = nx;
+          group = nx;
+        }
+        for (uint32_t s = 0; placed < cnt; s++) {
+          slot_key[(size_t)15 * group + s] = bk[placed];
+          unit.offset_[(size_t)32 * group + 2 * s] = ku[bk[placed++]];
+        }
+      }
+      kstart.resize((size_t)K + 1);
+      kstart[0] = 0;
+      for (uint32_t j = 0; j < K; j++) kstart[j + 1] = kstart[j] + key_len[u][j];
+      const uint32_t* src = lists[u].data();
+      uint32_t pos = 0;
+      for (uint32_t grp = 0; grp < K; grp++) {
+        for (uint32_t s = 0; s < 15; s++) {
+          const uint32_t j = slot_key[(size_t)15 * grp + s];
+          if (j == kInvalid) break;
+          unit.offset_[(size_t)32 * grp + 2 * s + 1] = pos;
+          const uint32_t len = key_len[u][j];
+          std::memcpy(unit.edges_.data() + pos, src + kstart[j], 4ull * len);
+          pos += len;
+          unit.offset_[(size_t)32 * grp + 2 * (s + 1) + 1] = pos;
+        }
+      }
+      std::vector<uint32_t>().swap(keys[u]);
+      std::vector<uint32_t>().swap(key_len[u]);
+      std::vector<uint32_t>().swap(lists[u]);
     }
+  };
+  const uint64_t nnz = off[n];
+  unsigned T = nnz > (1u << 20) ? std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1u;
+  T = std::min<unsigned>(T, std::max<uint32_t>(num_units, 1));
+  if (T == 1) {
+    worker();
+  } else {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; t++) th.emplace_back(worker);
+    for (auto& x : th) x.join();
   }
 }
 
@@ -220,25 +265,50 @@ std::vector<uint32_t> signature_from_csr(uint32_t n, const uint32_t* off, const 
                                          const int32_t* labels, bool column_oriented, bool x86_shift) {
   const uint32_t words = kSigWords;                // 16
   const uint32_t slots = (kSigBits - 32) / 2;      // 240 two-bit counters
-  std::vector<uint32_t> row((size_t)words * n, 0);
+  // slot of each neighbour label, hashed once per label when labels are small
+  int32_t max_label = -1;
+  bool nonneg = true;
   for (uint32_t v = 0; v < n; v++) {
-    uint32_t* sig = &row[(size_t)v * words];
-    sig[0] = (uint32_t)labels[v];
-    for (uint32_t i = off[v]; i < off[v + 1]; i++) {
-      int32_t nl = labels[adj[i]];
-      uint32_t h = murmur_hash2(&nl, 4, kHashSeed) % slots;
-      uint32_t word = 1 + (2 * h) / 32;
-      uint32_t bit = (2 * h) % 32;
-      uint32_t shift = x86_shift ? ((2 * bit) & 31u) : bit;
-      uint32_t cnt = (sig[word] >> shift) & 3u;
-      uint32_t next = cnt == 0 ? 1u : 3u;
-      sig[word] |= next << shift;
-    }
+    max_label = std::max(max_label, labels[v]);
+    nonneg = nonneg && labels[v] >= 0;
   }
+  std::vector<uint32_t> slot_of;
+  if (nonneg && max_label >= 0 && max_label < (1 << 20)) {
+    slot_of.resize((size_t)max_label + 1);
+    for (int32_t l = 0; l <= max_label; l++) slot_of[l] = murmur_hash2(&l, 4, kHashSeed) % slots;
+  }
+  std::vector<uint32_t> row((size_t)words * n, 0);
+  auto build = [&](uint32_t lo, uint32_t hi) {
+    for (uint32_t v = lo; v < hi; v++) {
+      uint32_t* sig = &row[(size_t)v * words];
+      sig[0] = (uint32_t)labels[v];
+      for (uint32_t i = off[v]; i < off[v + 1]; i++) {
+        int32_t nl = labels[adj[i]];
+        uint32_t h = slot_of.empty() ? murmur_hash2(&nl, 4, kHashSeed) % slots : slot_of[nl];
+        uint32_t word = 1 + (2 * h) / 32;
+        uint32_t bit = (2 * h) % 32;
+        uint32_t shift = x86_shift ? ((2 * bit) & 31u) : bit;
+        uint32_t cnt = (sig[word] >> shift) & 3u;
+        uint32_t next = cnt == 0 ? 1u : 3u;
+        sig[word] |= next << shift;
+      }
+    }
+  };
+  const unsigned T = off[n] > (1u << 20) ? std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1u;
+  auto parallel = [&](auto&& fn) {
+    if (T == 1) return fn(0u, n);
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; t++)
+      th.emplace_back(fn, (uint32_t)((uint64_t)n * t / T), (uint32_t)((uint64_t)n * (t + 1) / T));
+    for (auto& x : th) x.join();
+  };
+  parallel(build);
   if (!column_oriented) return row;
   std::vector<uint32_t> col((size_t)words * n);
-  for (uint32_t w = 0; w < words; w++)
-    for (uint32_t v = 0; v < n; v++) col[(size_t)w * n + v] = row[(size_t)v * words + w];
+  parallel([&](uint32_t lo, uint32_t hi) {
+    for (uint32_t w = 0; w < words; w++)
+      for (uint32_t v = lo; v < hi; v++) col[(size_t)w * n + v] = row[(size_t)v * words + w];
+  });
   return col;
 }
 

@@ -11,8 +11,13 @@
 //   PCSR units / then per unit: UNIT keys nedges / offset words / edges
 //   SIGQ (row-major [n][16]) / SIGD (column-major [16][n])
 //   MURMUR count / hashes of the 4-byte keys 0..count-1 with seed 17
+// With "time" as the third argument it prints only the wall time of each
+// stage the reference runs for alg 2 (`TIME <stage> <seconds>`: read, pcsr,
+// csr, signature), the host-ingest baseline.
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <string>
 
 #include "IO/graph_reader.h"
@@ -26,12 +31,38 @@ static void dump(const char* tag, const unsigned* a, size_t n) {
   printf("\n");
 }
 
+static int time_stages(const char* path, bool labeled) {
+  using clk = std::chrono::steady_clock;
+  auto secs = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+  auto t0 = clk::now();
+  UndirectedSurveyPlainReader reader;
+  Graph* g = nullptr;
+  reader.read(path, g, labeled);
+  UndirectedPlainGraph* plain = dynamic_cast<UndirectedPlainGraph*>(g);
+  auto t1 = clk::now();
+  Graph* pg = nullptr;
+  UndirectedPlainToPCSRTransformer pt;
+  pt.transform(plain, pg);
+  auto t2 = clk::now();
+  Graph* cg = nullptr;
+  UndirectedPlainToCSRTransformer ct;
+  ct.transform(plain, cg);
+  auto t3 = clk::now();
+  Vertex* sd = nullptr;
+  UndirectedPlainToPCSRTransformer::build_signature(true, sd, plain);
+  auto t4 = clk::now();
+  printf("TIME read %.6f\nTIME pcsr %.6f\nTIME csr %.6f\nTIME signature %.6f\n", secs(t0, t1), secs(t1, t2),
+         secs(t2, t3), secs(t3, t4));
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc < 3) {
-    fprintf(stderr, "usage: ref_dump <graph file> <labeled 0|1> [murmur_count]\n");
+    fprintf(stderr, "usage: ref_dump <graph file> <labeled 0|1> [murmur_count | time]\n");
     return 2;
   }
   const bool labeled = atoi(argv[2]) != 0;
+  if (argc > 3 && std::strcmp(argv[3], "time") == 0) return time_stages(argv[1], labeled);
   const unsigned murmur_count = argc > 3 ? (unsigned)atoi(argv[3]) : 64;
   UndirectedSurveyPlainReader reader;
   Graph* g = nullptr;

@@ -96,6 +96,24 @@ def test_generated_graph_matches_reference_on_file():
         assert np.array_equal(o, r["units"][u]["offset"]) and np.array_equal(e, r["units"][u]["edges"])
 
 
+def test_threaded_builders_match_reference_on_file(tmp_path):
+    # 1.2M adjacency entries: the PCSR and signature builders run on threads
+    # (above 2^20 entries); their output still equals the reference's own
+    if not O.ref_available():
+        pytest.skip("oracle/_ref not built")
+    g = F.HostGraph.chung_lu(60000, 600000, 2.3, 21).set_labels("zipf", 8, 1.2, 22)
+    assert g.nnz > (1 << 20)
+    path = str(tmp_path / "big.graph")
+    g.write(path)
+    r = O.ref_dump(path, True, 16)
+    assert np.array_equal(r["SIGD"], g.signature(True))
+    p = g.pcsr(reference_unit_ids=True)
+    assert p.num_units == len(r["units"])
+    for u in range(p.num_units):
+        o, e = p.unit(u)
+        assert np.array_equal(o, r["units"][u]["offset"]) and np.array_equal(e, r["units"][u]["edges"])
+
+
 def _write(td, text):
     path = os.path.join(td, "x.graph")
     with open(path, "w") as f:
