@@ -156,15 +156,44 @@ int read_plain_graph(const std::string& path, bool is_labeled, UndirectedPlainGr
     for (auto& e : sl.edges)
       if (e.first >= g.num_vertices_ || e.second >= g.num_vertices_) return kIoEdgeOutOfRange;
   // Pre-size adjacency from degrees so huge files do not re-grow vectors.
-  std::vector<uint32_t> deg(g.num_vertices_, 0);
-  for (auto& sl : slices)
-    for (auto& e : sl.edges) { deg[e.first]++; deg[e.second]++; }
-  g.edges_.assign(g.num_vertices_, {});
-  for (Vertex v = 0; v < g.num_vertices_; v++) g.edges_[v].reserve(deg[v]);
+  const Vertex n = g.num_vertices_;
+  std::vector<uint32_t> deg(n, 0);
+  uint64_t nedges = 0;
   for (auto& sl : slices) {
-    for (auto& e : sl.edges) g.add_edge(e.first, e.second);
-    std::vector<std::pair<Vertex, Vertex>>().swap(sl.edges);
+    nedges += sl.edges.size();
+    for (auto& e : sl.edges) { deg[e.first]++; deg[e.second]++; }
   }
+  g.edges_.assign(n, {});
+  for (Vertex v = 0; v < n; v++) g.edges_[v].reserve(deg[v]);
+  if (T == 1) {
+    for (auto& sl : slices)
+      for (auto& e : sl.edges) g.add_edge(e.first, e.second);
+  } else {
+    // Each thread owns a vertex range of about equal adjacency and appends,
+    // in file order, the entries of every edge with an endpoint in its range:
+    // every list comes out exactly as a sequential add_edge loop builds it.
+    std::vector<Vertex> lo(T + 1, n);
+    lo[0] = 0;
+    uint64_t acc = 0;
+    unsigned t = 1;
+    for (Vertex v = 0; v < n && t < T; v++) {
+      acc += deg[v];
+      while (t < T && acc >= 2 * nedges * t / T) lo[t++] = v + 1;
+    }
+    std::vector<std::thread> th;
+    for (unsigned w = 0; w < T; w++)
+      th.emplace_back([&, w] {
+        const Vertex a0 = lo[w], a1 = lo[w + 1];
+        for (auto& sl : slices)
+          for (auto& e : sl.edges) {
+            if (e.first >= a0 && e.first < a1) g.edges_[e.first].push_back(e.second);
+            if (e.second >= a0 && e.second < a1) g.edges_[e.second].push_back(e.first);
+          }
+      });
+    for (auto& x : th) x.join();
+    g.num_edges_ += (Edge)nedges;
+  }
+  for (auto& sl : slices) std::vector<std::pair<Vertex, Vertex>>().swap(sl.edges);
   g.finalize();
   g.set_min_degree();
   return kIoOk;

@@ -95,17 +95,44 @@ int UndirectedPlainToCSRTransformer::transform(const Graph* input, Graph*& outpu
   csr->is_labeled_ = g->is_labeled_;
   csr->labels_.assign(g->labels_.begin(), g->labels_.begin() + n);
   csr->offset_.assign((size_t)n + 1, 0);
-  std::vector<Vertex> scratch;
-  for (Vertex v = 0; v < n; v++) {
-    scratch.assign(g->edges_[v].begin(), g->edges_[v].end());
-    std::sort(scratch.begin(), scratch.end());
-    if (dedupe_) {
-      scratch.erase(std::unique(scratch.begin(), scratch.end()), scratch.end());
-      scratch.erase(std::remove(scratch.begin(), scratch.end(), v), scratch.end());
+  // Each list is copied to its raw position, sorted and (with dedupe) cleared
+  // of repeats and self-loops in place; the kept prefixes are then packed.
+  // Both passes split the vertices into ranges of about equal adjacency.
+  std::vector<uint64_t> raw((size_t)n + 1, 0);
+  for (Vertex v = 0; v < n; v++) raw[v + 1] = raw[v] + g->edges_[v].size();
+  const uint64_t total = raw[n];
+  std::vector<Vertex> tmp(total);
+  std::vector<Edge> len(n);
+  unsigned T = total > (1u << 20) ? std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1u;
+  std::vector<Vertex> lo(T + 1, n);
+  lo[0] = 0;
+  for (unsigned t = 1; t < T; t++)
+    lo[t] = (Vertex)(std::lower_bound(raw.begin(), raw.end(), total * t / T) - raw.begin());
+  for (unsigned t = 1; t <= T; t++) lo[t] = std::max(lo[t], lo[t - 1]);
+  auto run = [&](auto&& fn) {
+    if (T == 1) return fn(0u, n);
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; t++) th.emplace_back(fn, lo[t], lo[t + 1]);
+    for (auto& x : th) x.join();
+  };
+  run([&](Vertex a, Vertex b) {
+    for (Vertex v = a; v < b; v++) {
+      Vertex* p = tmp.data() + raw[v];
+      Vertex* e = std::copy(g->edges_[v].begin(), g->edges_[v].end(), p);
+      std::sort(p, e);
+      if (dedupe_) {
+        e = std::unique(p, e);
+        e = std::remove(p, e, v);
+      }
+      len[v] = (Edge)(e - p);
     }
-    csr->offset_[v + 1] = csr->offset_[v] + (Edge)scratch.size();
-    csr->edges_.insert(csr->edges_.end(), scratch.begin(), scratch.end());
-  }
+  });
+  for (Vertex v = 0; v < n; v++) csr->offset_[v + 1] = csr->offset_[v] + len[v];
+  csr->edges_.resize(csr->offset_[n]);
+  run([&](Vertex a, Vertex b) {
+    for (Vertex v = a; v < b; v++)
+      std::copy(tmp.data() + raw[v], tmp.data() + raw[v] + len[v], csr->edges_.data() + csr->offset_[v]);
+  });
   csr->num_edges_ = dedupe_ ? csr->offset_[n] / 2 : g->num_edges_;
   Vertex md = n;
   for (Vertex v = 0; v < n; v++) md = std::min(md, csr->degree(v));
