@@ -1,6 +1,8 @@
 """Benchmark of the MI355X subgraph-matching engine.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
+    python bench.py --workload c3|c4|c5      (other GPU workloads)
+    python bench.py --workload ingest        (host ingest, no GPU)
 
 A step is one pass of the hot path over the C2 workload (the dblp-shaped
 configuration the headline metric is quoted on): a Chung-Lu data graph of
@@ -106,7 +108,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "ingest"])
     ap.add_argument("--qps", type=float, default=0.0, help="c5: arrival rate (0: the workload's)")
     ap.add_argument("--requests", type=int, default=0, help="c5: stream length (0: the workload's)")
     ap.add_argument("--workers", type=int, default=4, help="c5: server worker contexts")
@@ -119,6 +121,9 @@ def main():
         print(json.dumps({"error": "this engine runs on one GPU: the hot path is single-device; "
                                    "--gpus N>1 is not supported"}))
         sys.exit(2)
+
+    if args.workload == "ingest":
+        return bench_ingest(args)
 
     import numpy as np
     import fsig_amd as F
@@ -371,6 +376,68 @@ def bench_c5(args):
            "embeddings": int(sum(plain_counts)),
            "server": {k: (round(v, 1) if isinstance(v, float) else v) for k, v in st.items()}}
     print(json.dumps(out), flush=True)
+
+
+def bench_ingest(args):
+    """Host ingest of the labeled workload graphs as text files (no GPU):
+    the parallel reader + CSR build, the PCSR (reference unit ids), the data
+    signature and the binary CSR cache round trip, each the best of
+    `--steps` runs.  `value` is text MB/s through read + CSR + PCSR +
+    signature on the C2 file; `cpu_baseline` is the reference's own reader,
+    transforms and signature (oracle/_ref/ref_dump, compiled from the
+    reference sources; kind "reference") on the same file.  The C4 file
+    (1M V / 10M E, 32 Zipf labels) is timed for this engine only: the
+    reference's O(K^2) PCSR build takes minutes there."""
+    import tempfile
+    import fsig_amd as F
+    from fsig_amd import workloads as W
+    import oracle as O
+    steps = max(1, min(args.steps, 5))
+    res = {}
+    with tempfile.TemporaryDirectory() as td:
+        for name, make in (("c2", W.c2_graph), ("c4", W.c4_graph)):
+            g = make()
+            path = os.path.join(td, f"{name}.graph")
+            g.write(path)
+            mb = os.path.getsize(path) / 1e6
+            best = {}
+
+            def timed(key, fn):
+                t = time.perf_counter()
+                out = fn()
+                best[key] = min(best.get(key, 1e30), time.perf_counter() - t)
+                return out
+
+            for _ in range(steps):
+                h = timed("read_csr", lambda: F.HostGraph.read(path))
+                timed("pcsr", lambda: h.pcsr(reference_unit_ids=True))
+                timed("signature", lambda: h.signature(True))
+                binp = os.path.join(td, f"{name}.csr.bin")
+                timed("bin_save", lambda: h.save_bin(binp))
+                timed("bin_load", lambda: F.HostGraph.load_bin(binp))
+            ours = best["read_csr"] + best["pcsr"] + best["signature"]
+            res[name] = {"text_MB": round(mb, 1), "vertices": g.n, "edges": g.nnz // 2,
+                         "seconds": {k: round(v, 4) for k, v in best.items()}, "MB_per_s": round(mb / ours, 1)}
+            log(name, res[name])
+            if name == "c2" and not args.no_cpu and O.ref_available():
+                t = O.ref_ingest_seconds(path, True)
+                ref_total = sum(t.values())
+                res["reference_c2"] = {"seconds": t, "MB_per_s": round(mb / ref_total, 1)}
+                log("reference", res["reference_c2"])
+    c2 = res["c2"]
+    out = {"metric": "host ingest MB/s (text -> CSR + PCSR + signature)", "value": c2["MB_per_s"],
+           "unit": "MB/s", "n_gpus": 0, "steps": steps, "warmup": 0,
+           "ms_per_step": round(1000 * (c2["seconds"]["read_csr"] + c2["seconds"]["pcsr"] + c2["seconds"]["signature"]), 1),
+           "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+           "config": {"workload": "ingest: C2 text file (317,080 V, 1,049,866 E, 16 Zipf labels); C4 file for this "
+                                  "engine only", "threads": min(16, os.cpu_count() or 1)},
+           "stages": res}
+    if "reference_c2" in res:
+        out["cpu_baseline"] = {"value": res["reference_c2"]["MB_per_s"], "unit": "MB/s", "cores": 1,
+                               "kind": "reference", "cpu_model": cpu_model(),
+                               "sample": "the reference's reader, PCSR and CSR transforms and data signature "
+                                         "(oracle/_ref/ref_dump time) on the same C2 file, one run"}
+    print(json.dumps(out))
 
 
 def bench_c3(args):

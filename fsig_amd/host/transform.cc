@@ -150,21 +150,47 @@ void pcsr_from_csr(uint32_t n, const uint32_t* off, const uint32_t* adj, const i
   };
   // Pass 1, one sweep of the adjacency: per unit its keys in ascending vertex
   // order, each key's list length, and the lists themselves back to back in
-  // key order (a key's list keeps its adjacency order, ascending).
-  std::vector<std::vector<uint32_t>> keys(num_units), key_len(num_units), lists(num_units);
-  std::vector<uint32_t> last_key(num_units, kInvalid);
-  for (uint32_t v = 0; v < n; v++) {
-    for (uint32_t i = off[v]; i < off[v + 1]; i++) {
-      const uint32_t u = unit_of(labels[v], labels[adj[i]]);
-      if (last_key[u] != v) {
-        last_key[u] = v;
-        keys[u].push_back(v);
-        key_len[u].push_back(0);
+  // key order (a key's list keeps its adjacency order, ascending).  Large
+  // graphs sweep contiguous vertex ranges on threads; a unit's pieces are
+  // joined in range order, which keeps the keys ascending.
+  const uint64_t nnz = off[n];
+  unsigned T = nnz > (1u << 20) ? std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1u;
+  T = std::min<unsigned>(T, std::max<uint32_t>(num_units, 1));
+  const unsigned T1 = (uint64_t)T * num_units <= (1u << 16) ? T : 1u;
+  struct Piece {
+    std::vector<uint32_t> keys, key_len, list;
+  };
+  std::vector<std::vector<Piece>> piece(T1, std::vector<Piece>(num_units));
+  auto sweep = [&](unsigned t) {
+    const uint32_t v0 = (uint32_t)(std::lower_bound(off, off + n + 1, nnz * t / T1) - off);
+    const uint32_t v1 = t + 1 == T1 ? n : (uint32_t)(std::lower_bound(off, off + n + 1, nnz * (t + 1) / T1) - off);
+    std::vector<Piece>& P = piece[t];
+    std::vector<uint32_t> last_key(num_units, kInvalid);
+    for (uint32_t v = v0; v < v1; v++) {
+      for (uint32_t i = off[v]; i < off[v + 1]; i++) {
+        const uint32_t u = unit_of(labels[v], labels[adj[i]]);
+        Piece& pc = P[u];
+        if (last_key[u] != v) {
+          last_key[u] = v;
+          pc.keys.push_back(v);
+          pc.key_len.push_back(0);
+        }
+        pc.key_len.back()++;
+        pc.list.push_back(adj[i]);
       }
-      key_len[u].back()++;
-      lists[u].push_back(adj[i]);
     }
+  };
+  if (T1 == 1) {
+    sweep(0);
+  } else {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T1; t++) th.emplace_back(sweep, t);
+    for (auto& x : th) x.join();
   }
+  std::vector<std::vector<uint32_t>> keys(num_units), key_len(num_units), lists(num_units);
+  std::vector<uint64_t> unit_edges(num_units, 0);
+  for (uint32_t u = 0; u < num_units; u++)
+    for (unsigned t = 0; t < T1; t++) unit_edges[u] += piece[t][u].list.size();
   units.assign(num_units, PCSRUnit());
   // Pass 2, units in parallel (largest first): hash the keys into K groups of
   // 15 slots (counting sort by bucket, insertion order kept within a bucket),
@@ -172,13 +198,25 @@ void pcsr_from_csr(uint32_t n, const uint32_t* off, const uint32_t* adj, const i
   // order, then copy each key's list into place group by group, slot by slot.
   std::vector<uint32_t> order(num_units);
   for (uint32_t u = 0; u < num_units; u++) order[u] = u;
-  std::sort(order.begin(), order.end(),
-            [&](uint32_t a, uint32_t b) { return lists[a].size() > lists[b].size(); });
+  std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return unit_edges[a] > unit_edges[b]; });
   std::atomic<uint32_t> next{0};
   auto worker = [&]() {
     std::vector<uint32_t> hash, bstart, byb, slot_key, kstart;
     for (uint32_t t = next++; t < num_units; t = next++) {
       const uint32_t u = order[t];
+      if (T1 == 1) {
+        keys[u].swap(piece[0][u].keys);
+        key_len[u].swap(piece[0][u].key_len);
+        lists[u].swap(piece[0][u].list);
+      } else {
+        for (unsigned r = 0; r < T1; r++) {
+          Piece& pc = piece[r][u];
+          keys[u].insert(keys[u].end(), pc.keys.begin(), pc.keys.end());
+          key_len[u].insert(key_len[u].end(), pc.key_len.begin(), pc.key_len.end());
+          lists[u].insert(lists[u].end(), pc.list.begin(), pc.list.end());
+          pc = Piece();
+        }
+      }
       PCSRUnit& unit = units[u];
       const uint32_t K = (uint32_t)keys[u].size();
       const uint64_t ne = lists[u].size();
@@ -250,9 +288,6 @@ void pcsr_from_csr(uint32_t n, const uint32_t* off, const uint32_t* adj, const i
       std::vector<uint32_t>().swap(lists[u]);
     }
   };
-  const uint64_t nnz = off[n];
-  unsigned T = nnz > (1u << 20) ? std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1u;
-  T = std::min<unsigned>(T, std::max<uint32_t>(num_units, 1));
   if (T == 1) {
     worker();
   } else {

@@ -147,6 +147,15 @@ def ref_available() -> bool:
     return os.path.exists(REF_DUMP)
 
 
+def ref_ingest_seconds(path: str, labeled: bool = True, timeout: float = 600.0) -> Dict[str, float]:
+    """Wall seconds of the reference's own ingest stages on a file (its
+    reader, PCSR transform, CSR transform and data signature), as timed by
+    ref_dump itself: {"read": s, "pcsr": s, "csr": s, "signature": s}."""
+    out = subprocess.run([REF_DUMP, path, "1" if labeled else "0", "time"], check=True, capture_output=True,
+                         text=True, timeout=timeout).stdout
+    return {f[1]: float(f[2]) for f in (line.split() for line in out.splitlines()) if len(f) == 3 and f[0] == "TIME"}
+
+
 def ref_dump(path: str, labeled: bool = True, murmur_count: int = 64) -> Dict[str, object]:
     """Runs the reference's own reader/CSR/PCSR/signature code on a file."""
     out = subprocess.run([REF_DUMP, path, "1" if labeled else "0", str(murmur_count)], check=True,
