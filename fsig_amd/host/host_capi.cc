@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <memory>
 #include <mutex>
+#include <thread>
 
 #include "gen.h"
 #include "graph.h"
@@ -68,6 +69,24 @@ struct BinFile {
 
 }  // namespace
 
+// Frees the adjacency lists of a large plain graph on several threads (one
+// free per vertex, millions of them at the largest sizes).
+static void release_lists(UndirectedPlainGraph& g) {
+  const size_t n = g.edges_.size();
+  const unsigned T = n > (1u << 20) ? std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1u;
+  auto drop = [&](size_t a, size_t b) {
+    for (size_t v = a; v < b; v++) std::vector<Vertex>().swap(g.edges_[v]);
+  };
+  if (T == 1) {
+    drop(0, n);
+  } else {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; t++) th.emplace_back(drop, n * t / T, n * (t + 1) / T);
+    for (auto& x : th) x.join();
+  }
+  std::vector<std::vector<Vertex>>().swap(g.edges_);
+}
+
 static void fix_num_labels(fsig_hgraph* g) {
   int32_t mx = 0;
   for (int32_t l : g->labels) mx = std::max(mx, l);
@@ -85,11 +104,12 @@ fsig_hgraph* fsig_hg_read(const char* path, int is_labeled, int* status) {
   UndirectedPlainToCSRTransformer t(true);
   if (t.transform(&plain, out)) return nullptr;
   std::unique_ptr<UndirectedCSRGraph> csr(static_cast<UndirectedCSRGraph*>(out));
+  release_lists(plain);
   auto* g = new fsig_hgraph();
   g->n = csr->num_vertices_;
-  g->off = csr->offset_;
-  g->adj = csr->edges_;
-  g->labels = csr->labels_;
+  g->off = std::move(csr->offset_);
+  g->adj = std::move(csr->edges_);
+  g->labels = std::move(csr->labels_);
   g->num_labels = std::max<int32_t>(csr->num_labels_, 1);
   return g;
 }

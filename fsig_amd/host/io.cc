@@ -8,6 +8,11 @@
 #include <memory>
 #include <thread>
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 namespace fsig {
 
 const char* io_status_string(int s) {
@@ -58,17 +63,36 @@ struct Cursor {
   }
 };
 
-bool slurp(const std::string& path, std::vector<char>& buf) {
-  FILE* f = fopen(path.c_str(), "rb");
-  if (!f) return false;
-  fseek(f, 0, SEEK_END);
-  long sz = ftell(f);
-  fseek(f, 0, SEEK_SET);
-  buf.resize(sz > 0 ? (size_t)sz : 0);
-  size_t got = sz > 0 ? fread(buf.data(), 1, (size_t)sz, f) : 0;
-  fclose(f);
-  return got == buf.size();
-}
+// Read-only mapping of a whole file: the parser threads page in their own
+// slices in parallel instead of one thread copying the file first.
+struct MappedFile {
+  const char* data = nullptr;
+  size_t size = 0;
+  bool ok = false;
+  explicit MappedFile(const std::string& path) {
+    const int fd = ::open(path.c_str(), O_RDONLY);
+    if (fd < 0) return;
+    struct stat st;
+    if (::fstat(fd, &st) == 0) {
+      size = (size_t)st.st_size;
+      if (size == 0) {
+        ok = true;
+      } else {
+        void* p = ::mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+        if (p != MAP_FAILED) {
+          data = (const char*)p;
+          ok = true;
+        }
+      }
+    }
+    ::close(fd);
+  }
+  ~MappedFile() {
+    if (data) ::munmap((void*)data, size);
+  }
+  MappedFile(const MappedFile&) = delete;
+  MappedFile& operator=(const MappedFile&) = delete;
+};
 
 }  // namespace
 
@@ -109,9 +133,9 @@ void parse_slice(const char* b, const char* e, bool is_labeled, ParsedSlice& out
 // slices are applied in file order, so the result and the first error
 // reported are those of a sequential parse.
 int read_plain_graph(const std::string& path, bool is_labeled, UndirectedPlainGraph& g) {
-  std::vector<char> buf;
-  if (!slurp(path, buf)) return kIoCannotOpen;
-  Cursor c{buf.data(), buf.data() + buf.size()};
+  MappedFile file(path);
+  if (!file.ok) return kIoCannotOpen;
+  Cursor c{file.data, file.data + file.size};
   char type;
   uint64_t hn, hm;
   if (!c.next_char(type) || type != 't') return kIoBadHeader;
@@ -119,7 +143,7 @@ int read_plain_graph(const std::string& path, bool is_labeled, UndirectedPlainGr
   g = UndirectedPlainGraph();
   g.is_labeled_ = is_labeled;
   const char* body = c.p;
-  const char* end = buf.data() + buf.size();
+  const char* end = file.data + file.size;
   const size_t len = (size_t)(end - body);
   unsigned T = 1;
   if (len > (16u << 20)) T = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -159,12 +183,32 @@ int read_plain_graph(const std::string& path, bool is_labeled, UndirectedPlainGr
   const Vertex n = g.num_vertices_;
   std::vector<uint32_t> deg(n, 0);
   uint64_t nedges = 0;
-  for (auto& sl : slices) {
-    nedges += sl.edges.size();
-    for (auto& e : sl.edges) { deg[e.first]++; deg[e.second]++; }
-  }
+  for (auto& sl : slices) nedges += sl.edges.size();
   g.edges_.assign(n, {});
-  for (Vertex v = 0; v < n; v++) g.edges_[v].reserve(deg[v]);
+  if (T == 1) {
+    for (auto& sl : slices)
+      for (auto& e : sl.edges) { deg[e.first]++; deg[e.second]++; }
+    for (Vertex v = 0; v < n; v++) g.edges_[v].reserve(deg[v]);
+  } else {
+    // degrees: each slice's thread counts its own edges with relaxed atomic
+    // increments; then the lists are reserved by vertex ranges in parallel
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; t++)
+      th.emplace_back([&, t] {
+        for (auto& e : slices[t].edges) {
+          __atomic_fetch_add(&deg[e.first], 1u, __ATOMIC_RELAXED);
+          __atomic_fetch_add(&deg[e.second], 1u, __ATOMIC_RELAXED);
+        }
+      });
+    for (auto& x : th) x.join();
+    th.clear();
+    for (unsigned t = 0; t < T; t++)
+      th.emplace_back([&, t] {
+        const Vertex a = (Vertex)((uint64_t)n * t / T), b = (Vertex)((uint64_t)n * (t + 1) / T);
+        for (Vertex v = a; v < b; v++) g.edges_[v].reserve(deg[v]);
+      });
+    for (auto& x : th) x.join();
+  }
   if (T == 1) {
     for (auto& sl : slices)
       for (auto& e : sl.edges) g.add_edge(e.first, e.second);

@@ -139,6 +139,12 @@ def test_reader_errors():
         with pytest.raises(F.FsigError) as e:
             F.HostGraph.read(_write(td, "x 1 2\n"))
         assert e.value.code == -2
+        with pytest.raises(F.FsigError) as e:  # empty file: no header
+            F.HostGraph.read(_write(td, ""))
+        assert e.value.code == -2
+        with pytest.raises(F.FsigError) as e:  # a directory cannot be read as a graph
+            F.HostGraph.read(td)
+        assert e.value.code == -1
         with pytest.raises(F.FsigError) as e:  # id 1 has no v line
             F.HostGraph.read(_write(td, "t 3 1\nv 0 0 1\nv 2 0 1\ne 0 2"))
         assert e.value.code == -4
