@@ -97,8 +97,17 @@ def cpu_baseline(g, queries, seconds_target=20.0):
             parts.append(f"{name}: {'first ' if stopped else 'all '}{cnt} embeddings")
             if not stopped:
                 checks[name] = cnt
+    # one thread on the sparse query, bounded to about an eighth of the budget
+    name, q = queries[-1]
+    rate1 = total_emb / total_s / threads
+    limit1 = max(1_000_000, int(rate1 * seconds_target / 8))
+    t = time.perf_counter()
+    cnt1, _, stopped1 = O.match_partial(q.labels, q.adjm(), off, adj, lab, threads=1, limit=limit1)
+    dt1 = time.perf_counter() - t
     return {"value": total_emb / total_s, "unit": "embeddings/s", "cores": threads, "kind": "port",
             "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "single_thread_value": cnt1 / max(dt1, 1e-9),
+            "single_thread_sample": f"{name}: {'first ' if stopped1 else 'all '}{cnt1} embeddings in {dt1:.1f} s",
             "sample": "oracle backtracking (oracle/oracle.c), roots handed out dynamically to "
                       f"{threads} threads, {total_s:.1f} s: " + "; ".join(parts)}, checks
 

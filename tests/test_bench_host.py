@@ -25,10 +25,10 @@ def test_cpu_baseline_fields_and_counts():
     g.set_labels("uniform", 4, seed=1)
     qs = [("dense", g.walk_query(4, 5, 6, seed=3)), ("sparse", g.walk_query(5, 4, 4, seed=4))]
     cb, checks = bench.cpu_baseline(g, qs, seconds_target=0.5)
-    for key in ("value", "unit", "cores", "kind", "sample", "cpu_model"):
+    for key in ("value", "unit", "cores", "kind", "sample", "cpu_model", "single_thread_value"):
         assert key in cb
     assert cb["kind"] == "port" and cb["unit"] == "embeddings/s" and cb["cores"] >= 1
-    assert cb["value"] > 0
+    assert cb["value"] > 0 and cb["single_thread_value"] > 0
     assert "dense" in checks
     off, adj, lab = g.arrays()
     for name, q in qs:
