@@ -48,7 +48,11 @@ struct Cursor {
     skip_ws();
     if (p >= end || *p < '0' || *p > '9') return false;
     uint64_t x = 0;
-    while (p < end && *p >= '0' && *p <= '9') x = x * 10 + (uint64_t)(*p++ - '0');
+    while (p < end && *p >= '0' && *p <= '9') {
+      const uint64_t d = (uint64_t)(*p++ - '0');
+      if (x > (UINT64_MAX - d) / 10) return false;  // a number past 2^64 - 1 is a malformed record
+      x = x * 10 + d;
+    }
     v = x;
     return true;
   }

@@ -139,6 +139,9 @@ def test_reader_errors():
         with pytest.raises(F.FsigError) as e:
             F.HostGraph.read(_write(td, "x 1 2\n"))
         assert e.value.code == -2
+        with pytest.raises(F.FsigError) as e:  # 2^64 + 1 must not wrap to vertex 1
+            F.HostGraph.read(_write(td, "t 2 1\nv 0 0 1\nv 1 0 1\ne 0 18446744073709551617"))
+        assert e.value.code == -3
         with pytest.raises(F.FsigError) as e:  # empty file: no header
             F.HostGraph.read(_write(td, ""))
         assert e.value.code == -2
