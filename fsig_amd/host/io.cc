@@ -282,25 +282,84 @@ int write_graph(const std::string& path, Vertex n, const Label* labels, const Ed
   return kIoOk;
 }
 
+// Rows are sorted lexicographically (an index sort: chunks on threads, then
+// pairwise merges), gathered into sorted order, and printed block by block,
+// each block formatted on threads and written in order.
 int write_embeddings(const std::string& path, uint32_t k, uint64_t count, std::vector<uint32_t>& rows) {
   FILE* f = fopen(path.c_str(), "wb");
   if (!f) return kIoCannotOpen;
-  size_t n = k ? rows.size() / k : 0;
-  std::vector<size_t> idx(n);
+  const size_t n = k ? rows.size() / k : 0;
+  const unsigned T = n > (1u << 18) ? std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1u;
+  auto run = [&](unsigned parts, auto&& fn) {
+    if (parts == 1) return fn(0u);
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < parts; t++) th.emplace_back(fn, t);
+    for (auto& x : th) x.join();
+  };
+  const uint32_t* R = rows.data();
+  auto less = [&](size_t a, size_t b) {
+    const uint32_t* x = R + a * k;
+    const uint32_t* y = R + b * k;
+    for (uint32_t j = 0; j < k; j++)
+      if (x[j] != y[j]) return x[j] < y[j];
+    return false;
+  };
+  std::vector<size_t> idx(n), tmp;
   for (size_t i = 0; i < n; i++) idx[i] = i;
-  std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
-    return std::lexicographical_compare(rows.begin() + a * k, rows.begin() + (a + 1) * k,
-                                        rows.begin() + b * k, rows.begin() + (b + 1) * k);
-  });
-  fprintf(f, "#embedding: %llu\n", (unsigned long long)count);
+  std::vector<size_t> cut(T + 1);
+  for (unsigned t = 0; t <= T; t++) cut[t] = n * t / T;
+  run(T, [&](unsigned t) { std::sort(idx.begin() + cut[t], idx.begin() + cut[t + 1], less); });
+  if (T > 1) {
+    tmp.resize(n);
+    for (unsigned width = 1; width < T; width *= 2) {
+      const unsigned pairs = (T + 2 * width - 1) / (2 * width);
+      run(pairs, [&](unsigned pr) {
+        const unsigned a = pr * 2 * width, m = std::min(T, a + width), b = std::min(T, a + 2 * width);
+        std::merge(idx.begin() + cut[a], idx.begin() + cut[m], idx.begin() + cut[m], idx.begin() + cut[b],
+                   tmp.begin() + cut[a], less);
+      });
+      idx.swap(tmp);
+    }
+  }
   std::vector<uint32_t> sorted(rows.size());
-  for (size_t i = 0; i < n; i++) {
-    std::memcpy(&sorted[i * k], &rows[idx[i] * k], k * sizeof(uint32_t));
-    for (uint32_t j = 0; j < k; j++) fprintf(f, j + 1 < k ? "%u " : "%u\n", sorted[i * k + j]);
+  run(T, [&](unsigned t) {
+    for (size_t i = cut[t]; i < cut[t + 1]; i++) std::memcpy(&sorted[i * k], R + idx[i] * k, k * sizeof(uint32_t));
+  });
+  std::vector<size_t>().swap(idx);
+  std::vector<size_t>().swap(tmp);
+  fprintf(f, "#embedding: %llu\n", (unsigned long long)count);
+  // decimal text, at most 11 bytes per number (10 digits and a separator)
+  const size_t block = (size_t)1 << 20;
+  std::vector<std::vector<char>> buf(T);
+  bool ok = true;
+  for (size_t b0 = 0; b0 < n && ok; b0 += block) {
+    const size_t b1 = std::min(n, b0 + block);
+    run(T, [&](unsigned t) {
+      const size_t r0 = b0 + (b1 - b0) * t / T, r1 = b0 + (b1 - b0) * (t + 1) / T;
+      std::vector<char>& out = buf[t];
+      out.resize((r1 - r0) * k * 11);
+      char* o = out.data();
+      for (size_t i = r0; i < r1; i++) {
+        for (uint32_t j = 0; j < k; j++) {
+          uint32_t v = sorted[i * k + j];
+          char d[10];
+          int nd = 0;
+          do {
+            d[nd++] = (char)('0' + v % 10);
+            v /= 10;
+          } while (v);
+          while (nd) *o++ = d[--nd];
+          *o++ = j + 1 < k ? ' ' : '\n';
+        }
+      }
+      out.resize((size_t)(o - out.data()));
+    });
+    for (unsigned t = 0; t < T && ok; t++)
+      ok = buf[t].empty() || fwrite(buf[t].data(), 1, buf[t].size(), f) == buf[t].size();
   }
   rows.swap(sorted);
-  fclose(f);
-  return kIoOk;
+  if (fclose(f) != 0) ok = false;
+  return ok ? kIoOk : kIoCannotOpen;
 }
 
 }  // namespace fsig

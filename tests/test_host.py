@@ -232,3 +232,23 @@ def test_pcsr_label_cap():
     with pytest.raises(F.FsigError):
         g.pcsr()
     assert F.HostGraph.chung_lu(600, 2000, 2.3, 3).set_labels("uniform", 64, seed=2).pcsr().num_units == 64 * 65 // 2
+
+
+@pytest.mark.parametrize("n", [0, 1, 1000, 300000])
+def test_embedding_file_is_sorted_canonical_text(tmp_path, n):
+    # fsig_write_embeddings: "#embedding: <count>" then every row, sorted
+    # lexicographically, numbers separated by one space (300,000 rows take
+    # the threaded sort and formatting)
+    import ctypes as C
+    k = 4
+    rng = np.random.default_rng(n + 1)
+    rows = rng.integers(0, 1 << 32, size=(n, k), dtype=np.uint64).astype(np.uint32)
+    if n > 10:
+        rows[: n // 10] = rows[0]                # equal rows
+        rows[n // 10: n // 5, :2] = rows[1, :2]  # equal prefixes
+    path = str(tmp_path / "emb.txt")
+    rc = F.host_lib().fsig_write_embeddings(path.encode(), k, 12345, rows.ctypes.data_as(C.POINTER(C.c_uint32)), n)
+    assert rc == 0
+    want = "#embedding: 12345\n" + "".join(" ".join(str(int(x)) for x in r) + "\n"
+                                          for r in sorted(map(tuple, rows.tolist())))
+    assert open(path).read() == want
