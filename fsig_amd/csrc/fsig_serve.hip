@@ -365,7 +365,10 @@ extern "C" int fsig_server_get_stats(fsig_server* s, fsig_server_stats* out) {
   if (!s || !out) return FSIG_EINVAL;
   std::lock_guard<std::mutex> lk(s->mu);
   *out = s->st;
-  out->wall_ms = s->started ? std::chrono::duration<double, std::milli>(s->t_last - s->t_first).count() : 0;
+  // first submit to last completion; 0 until a batch has completed
+  out->wall_ms = s->started && s->st.completed
+                     ? std::chrono::duration<double, std::milli>(s->t_last - s->t_first).count()
+                     : 0;
   return FSIG_OK;
 }
 

@@ -212,7 +212,7 @@ typedef struct fsig_server_stats {
   uint32_t max_concurrent; /* most batches in flight at once */
   uint32_t pad;
   double busy_ms;          /* summed batch wall time over the workers */
-  double wall_ms;          /* first submit to last completion */
+  double wall_ms;          /* first submit to last completion (0 before one) */
 } fsig_server_stats;
 int fsig_server_create(int device, const fsig_graph* g, uint32_t workers, uint64_t hbm_budget_bytes,
                        uint32_t max_batch, const fsig_tune* tune, fsig_server** out);
