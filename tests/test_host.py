@@ -252,3 +252,43 @@ def test_embedding_file_is_sorted_canonical_text(tmp_path, n):
     want = "#embedding: 12345\n" + "".join(" ".join(str(int(x)) for x in r) + "\n"
                                           for r in sorted(map(tuple, rows.tolist())))
     assert open(path).read() == want
+
+
+def test_threaded_pcsr_of_an_unlabeled_graph_holds_every_list():
+    # one unit and more than 2^20 adjacency entries: the threaded sweep with a
+    # single unit; every sampled key's list is its whole adjacency
+    g = F.HostGraph.rmat(16, 600000, seed=5)
+    assert g.nnz > (1 << 20)
+    off, adj, _ = g.arrays()
+    p = g.pcsr(reference_unit_ids=True)
+    assert p.num_units == 1
+    o, e = p.unit(0)
+    rng = np.random.default_rng(3)
+    for v in rng.choice(g.n, 300, replace=False):
+        got = _pcsr_lookup(o, e, int(v))
+        assert (list(got) if got is not None else []) == adj[off[v]:off[v + 1]].tolist()
+
+
+def test_signature_with_large_label_values_equals_the_oracle():
+    # labels at or past 2^20 take the path without a per-label slot table
+    rng = np.random.default_rng(4)
+    n = 3000
+    edges = rng.integers(0, n, size=(15000, 2))
+    edges = edges[edges[:, 0] != edges[:, 1]]
+    labels = (rng.integers(0, 50, size=n) * 100003 + (1 << 20)).astype(np.int32)
+    g = F.HostGraph.from_edges(n, edges, labels)
+    off, adj, lab = g.arrays()
+    for col in (True, False):
+        assert np.array_equal(g.signature(col), O.signature(off, adj, lab, col))
+
+
+@pytest.mark.parametrize("k", [1, 32])
+def test_embedding_file_widths(tmp_path, k):
+    import ctypes as C
+    rng = np.random.default_rng(k)
+    rows = rng.integers(0, 1000, size=(5000, k), dtype=np.uint64).astype(np.uint32)
+    path = str(tmp_path / "emb.txt")
+    rc = F.host_lib().fsig_write_embeddings(path.encode(), k, 5000, rows.ctypes.data_as(C.POINTER(C.c_uint32)), 5000)
+    assert rc == 0
+    want = "#embedding: 5000\n" + "".join(" ".join(map(str, r)) + "\n" for r in sorted(map(tuple, rows.tolist())))
+    assert open(path).read() == want
