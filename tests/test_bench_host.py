@@ -34,3 +34,28 @@ def test_cpu_baseline_fields_and_counts():
     for name, q in qs:
         if name in checks:
             assert checks[name] == O.match(q.labels, q.adjm(), off, adj, lab)[0]
+
+
+def test_ingest_workload_line(monkeypatch, capsys):
+    # the ingest workload on small graphs: one JSON line with the contract's
+    # fields, every stage timed, the reference's stages as the baseline when
+    # oracle/_ref is built
+    import argparse
+    import bench
+    import fsig_amd as F
+    import oracle as O
+    from fsig_amd import workloads as W
+    monkeypatch.setattr(W, "c2_graph", lambda: F.HostGraph.chung_lu(3000, 12000, 2.3, 1).set_labels("zipf", 4, 1.2, 2))
+    monkeypatch.setattr(W, "c4_graph", lambda: F.HostGraph.chung_lu(5000, 20000, 2.3, 3).set_labels("zipf", 8, 1.2, 4))
+    bench.bench_ingest(argparse.Namespace(steps=1, no_cpu=False, ingest_c5=False))
+    line = json.loads(capsys.readouterr().out.strip().splitlines()[-1])
+    for key in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better",
+                "scaling", "vs_baseline", "dtype", "data", "config"):
+        assert key in line
+    assert line["unit"] == "MB/s" and line["value"] > 0
+    for name in ("c2", "c4"):
+        secs = line["stages"][name]["seconds"]
+        assert set(secs) == {"read_csr", "pcsr", "signature", "bin_save", "bin_load"}
+    if O.ref_available():
+        assert line["cpu_baseline"]["kind"] == "reference"
+        assert set(line["stages"]["reference_c2"]["seconds"]) == {"read", "pcsr", "csr", "signature"}
