@@ -49,18 +49,35 @@ class EdgeSet {
     mask_ = cap - 1;
   }
   bool insert(uint64_t key) {  // key != 0
-    uint64_t h = key * 0x9e3779b97f4a7c15ull;
-    uint64_t i = (h >> 17) & mask_;
+    if (2 * (count_ + 1) > slots_.size()) grow();  // stays at most half full
+    uint64_t i = slot(key);
     while (true) {
       uint64_t cur = slots_[i];
       if (cur == key) return false;
-      if (cur == 0) { slots_[i] = key; return true; }
+      if (cur == 0) {
+        slots_[i] = key;
+        count_++;
+        return true;
+      }
       i = (i + 1) & mask_;
     }
   }
  private:
+  uint64_t slot(uint64_t key) const { return ((key * 0x9e3779b97f4a7c15ull) >> 17) & mask_; }
+  void grow() {
+    std::vector<uint64_t> old(2 * slots_.size(), 0);
+    old.swap(slots_);
+    mask_ = slots_.size() - 1;
+    for (uint64_t key : old) {
+      if (!key) continue;
+      uint64_t i = slot(key);
+      while (slots_[i]) i = (i + 1) & mask_;
+      slots_[i] = key;
+    }
+  }
   std::vector<uint64_t> slots_;
   uint64_t mask_;
+  uint64_t count_ = 0;
 };
 
 inline uint64_t pair_key(uint32_t u, uint32_t v) {
@@ -148,7 +165,22 @@ EdgeList gen_rmat(uint32_t scale, uint64_t m, double a, double b, double c, uint
   // Candidates are drawn in fixed chunks, each from its own stream, and then
   // filtered in chunk order, so the result does not depend on `threads`.
   const uint64_t chunk = 1u << 20;
-  EdgeSet seen(m + m / 4);
+  // Large graphs keep their keys in 16 shards.  Every round but the last
+  // inserts only keys it accepts, and the last inserts at most one round of
+  // candidates, so m + threads * chunk keys bound what the shards ever hold;
+  // each table has room for twice its share of that bound.  Small graphs keep
+  // one set, filled only until m edges are accepted.
+  const uint32_t nshards = m >= (1u << 22) ? 16u : 1u;
+  std::vector<EdgeSet> shards;
+  shards.reserve(nshards);
+  if (nshards == 1)
+    shards.emplace_back(m + m / 4);
+  else
+    for (uint32_t sh = 0; sh < nshards; sh++) shards.emplace_back((m + (uint64_t)threads * chunk) / nshards + 1024);
+  auto shard_of = [&](uint64_t key) -> uint32_t {
+    return nshards == 1 ? 0u : (uint32_t)((key * 0xd6e8feb86659fd93ull) >> 60);
+  };
+  std::vector<uint8_t> flags;
   EdgeList edges;
   edges.reserve(m);
   uint64_t next_chunk = 0;
@@ -179,15 +211,53 @@ EdgeList gen_rmat(uint32_t scale, uint64_t m, double a, double b, double c, uint
     }
     for (auto& th : pool) th.join();
     next_chunk += threads;
-    for (int t = 0; t < threads && edges.size() < m; t++)
-      for (uint64_t x : bufs[t]) {
-        uint32_t u = (uint32_t)(x >> 32), v = (uint32_t)x;
-        if (u == v) continue;
-        if (seen.insert(pair_key(u, v))) {
-          edges.emplace_back(std::min(u, v), std::max(u, v));
-          if (edges.size() == m) break;
+    // Flag each candidate that is the first of its key so far, in candidate
+    // order.  Large graphs split the key set into shards by hash, one thread
+    // per group of shards, each thread scanning the candidates in order and
+    // handling its shards' keys: a key's candidates all meet in one shard in
+    // the same order, so the flags are those of one set.
+    if (nshards == 1) {
+      for (int t = 0; t < threads && edges.size() < m; t++)
+        for (uint64_t x : bufs[t]) {
+          const uint32_t u = (uint32_t)(x >> 32), v = (uint32_t)x;
+          if (u == v) continue;
+          if (shards[0].insert(pair_key(u, v))) {
+            edges.emplace_back(std::min(u, v), std::max(u, v));
+            if (edges.size() == m) break;
+          }
+        }
+      continue;
+    }
+    flags.assign((uint64_t)threads * chunk, 0);
+    auto mark = [&](int t, int nt) {
+      for (int b = 0; b < threads; b++) {
+        const uint64_t* buf = bufs[b].data();
+        uint8_t* fl = flags.data() + (uint64_t)b * chunk;
+        for (uint64_t i = 0; i < chunk; i++) {
+          const uint32_t u = (uint32_t)(buf[i] >> 32), v = (uint32_t)buf[i];
+          if (u == v) continue;
+          const uint64_t key = pair_key(u, v);
+          const uint32_t sh = shard_of(key);
+          if ((int)(sh % (uint32_t)nt) != t) continue;
+          fl[i] = shards[sh].insert(key) ? 1 : 0;
         }
       }
+    };
+    {
+      std::vector<std::thread> mk;
+      const int nt = std::min<int>(threads, (int)nshards);
+      for (int t = 0; t < nt; t++) mk.emplace_back(mark, t, nt);
+      for (auto& th : mk) th.join();
+    }
+    for (int t = 0; t < threads && edges.size() < m; t++) {
+      const uint8_t* fl = flags.data() + (uint64_t)t * chunk;
+      for (uint64_t i = 0; i < chunk; i++) {
+        if (!fl[i]) continue;
+        const uint32_t u = (uint32_t)(bufs[t][i] >> 32), v = (uint32_t)bufs[t][i];
+        edges.emplace_back(std::min(u, v), std::max(u, v));
+        if (edges.size() == m) break;
+      }
+    }
   }
   std::vector<uint32_t> perm = shuffled_ids(n, seed);
   for (auto& e : edges) e = {perm[e.first], perm[e.second]};

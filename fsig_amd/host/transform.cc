@@ -52,37 +52,69 @@ static void sort_ranges(uint32_t n, const std::vector<uint32_t>& off, std::vecto
   for (auto& th : pool) th.join();
 }
 
+// Every list ends sorted and free of repeats, so the scatter may fill a list
+// in any order.  Large edge lists are counted by threads over edge ranges
+// (relaxed atomic degree counts) and scattered by threads that own vertex
+// ranges; then each list is sorted, cleared of repeats in place and packed,
+// by threads over vertex ranges.
 void csr_from_edge_list(uint32_t n, const std::vector<std::pair<uint32_t, uint32_t>>& edges,
                         std::vector<uint32_t>& offset, std::vector<uint32_t>& adj, int threads) {
+  if (threads <= 0) threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const uint64_t ne = edges.size();
+  const int T = ne >= (1u << 20) ? threads : 1;
+  auto run = [&](auto&& fn) {
+    if (T == 1) return fn(0);
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; t++) th.emplace_back(fn, t);
+    for (auto& x : th) x.join();
+  };
   std::vector<uint32_t> deg((size_t)n + 1, 0);
-  for (auto& e : edges) {
-    if (e.first == e.second) continue;
-    deg[e.first]++;
-    deg[e.second]++;
-  }
+  run([&](int t) {
+    for (uint64_t i = ne * t / T; i < ne * (t + 1) / T; i++) {
+      const auto& e = edges[i];
+      if (e.first == e.second) continue;
+      __atomic_fetch_add(&deg[e.first], 1u, __ATOMIC_RELAXED);
+      __atomic_fetch_add(&deg[e.second], 1u, __ATOMIC_RELAXED);
+    }
+  });
+  std::vector<uint32_t> raw((size_t)n + 1, 0);
+  for (uint32_t v = 0; v < n; v++) raw[v + 1] = raw[v] + deg[v];
+  std::vector<uint32_t> lists(raw[n]);
+  std::vector<uint32_t> cur(raw.begin(), raw.end() - 1);
+  // each thread owns a vertex range of about equal adjacency and writes only
+  // its own vertices' entries (hub cursors are never shared)
+  std::vector<uint32_t> own(T + 1, n);
+  own[0] = 0;
+  for (int t = 1; t < T; t++)
+    own[t] = (uint32_t)(std::lower_bound(raw.begin(), raw.end(), (uint32_t)((uint64_t)raw[n] * t / T)) - raw.begin());
+  for (int t = 1; t <= T; t++) own[t] = std::max(own[t], own[t - 1]);
+  run([&](int t) {
+    const uint32_t a0 = own[t], a1 = own[t + 1];
+    for (const auto& e : edges) {
+      if (e.first == e.second) continue;
+      if (e.first >= a0 && e.first < a1) lists[cur[e.first]++] = e.second;
+      if (e.second >= a0 && e.second < a1) lists[cur[e.second]++] = e.first;
+    }
+  });
+  std::vector<uint32_t>().swap(cur);
+  sort_ranges(n, raw, lists, threads);
+  // repeats out, in place per list, then the kept prefixes packed
+  auto vrange = [&](int t) {
+    return std::make_pair((uint32_t)((uint64_t)n * t / T), (uint32_t)((uint64_t)n * (t + 1) / T));
+  };
+  run([&](int t) {
+    const auto r = vrange(t);
+    for (uint32_t v = r.first; v < r.second; v++)
+      deg[v] = (uint32_t)(std::unique(lists.begin() + raw[v], lists.begin() + raw[v + 1]) - (lists.begin() + raw[v]));
+  });
   offset.assign((size_t)n + 1, 0);
   for (uint32_t v = 0; v < n; v++) offset[v + 1] = offset[v] + deg[v];
   adj.resize(offset[n]);
-  std::vector<uint32_t> cur(offset.begin(), offset.end() - 1);
-  for (auto& e : edges) {
-    if (e.first == e.second) continue;
-    adj[cur[e.first]++] = e.second;
-    adj[cur[e.second]++] = e.first;
-  }
-  sort_ranges(n, offset, adj, threads);
-  // squeeze out duplicates
-  uint32_t w = 0;
-  uint32_t start = 0;
-  for (uint32_t v = 0; v < n; v++) {
-    uint32_t end = offset[v + 1];
-    uint32_t begin_w = w;
-    for (uint32_t i = start; i < end; i++)
-      if (w == begin_w || adj[w - 1] != adj[i]) adj[w++] = adj[i];
-    offset[v] = begin_w;
-    start = end;
-  }
-  offset[n] = w;
-  adj.resize(w);
+  run([&](int t) {
+    const auto r = vrange(t);
+    for (uint32_t v = r.first; v < r.second; v++)
+      std::copy(lists.begin() + raw[v], lists.begin() + raw[v] + deg[v], adj.begin() + offset[v]);
+  });
 }
 
 int UndirectedPlainToCSRTransformer::transform(const Graph* input, Graph*& output) {
