@@ -173,14 +173,22 @@ EdgeList gen_rmat(uint32_t scale, uint64_t m, double a, double b, double c, uint
   const uint32_t nshards = m >= (1u << 22) ? 16u : 1u;
   std::vector<EdgeSet> shards;
   shards.reserve(nshards);
-  if (nshards == 1)
+  if (nshards == 1) {
     shards.emplace_back(m + m / 4);
-  else
-    for (uint32_t sh = 0; sh < nshards; sh++) shards.emplace_back((m + (uint64_t)threads * chunk) / nshards + 1024);
+  } else {
+    for (uint32_t sh = 0; sh < nshards; sh++) shards.emplace_back(16);
+    std::vector<std::thread> al;
+    for (uint32_t sh = 0; sh < nshards; sh++)
+      al.emplace_back([&, sh] { shards[sh] = EdgeSet((m + (uint64_t)threads * chunk) / nshards + 1024); });
+    for (auto& th : al) th.join();
+  }
   auto shard_of = [&](uint64_t key) -> uint32_t {
     return nshards == 1 ? 0u : (uint32_t)((key * 0xd6e8feb86659fd93ull) >> 60);
   };
   std::vector<uint8_t> flags;
+  // byshard[t][sh]: positions in chunk t of the non-loop candidates of shard sh, in order
+  std::vector<std::vector<std::vector<uint32_t>>> byshard(
+      nshards == 1 ? 0 : threads, std::vector<std::vector<uint32_t>>(nshards));
   EdgeList edges;
   edges.reserve(m);
   uint64_t next_chunk = 0;
@@ -207,6 +215,13 @@ EdgeList gen_rmat(uint32_t scale, uint64_t m, double a, double b, double c, uint
           }
           buf[i] = (uint64_t)u << 32 | v;
         }
+        if (nshards > 1) {
+          for (auto& l : byshard[t]) l.clear();
+          for (uint64_t i = 0; i < chunk; i++) {
+            const uint32_t u = (uint32_t)(buf[i] >> 32), v = (uint32_t)buf[i];
+            if (u != v) byshard[t][shard_of(pair_key(u, v))].push_back((uint32_t)i);
+          }
+        }
       });
     }
     for (auto& th : pool) th.join();
@@ -230,18 +245,15 @@ EdgeList gen_rmat(uint32_t scale, uint64_t m, double a, double b, double c, uint
     }
     flags.assign((uint64_t)threads * chunk, 0);
     auto mark = [&](int t, int nt) {
-      for (int b = 0; b < threads; b++) {
-        const uint64_t* buf = bufs[b].data();
-        uint8_t* fl = flags.data() + (uint64_t)b * chunk;
-        for (uint64_t i = 0; i < chunk; i++) {
-          const uint32_t u = (uint32_t)(buf[i] >> 32), v = (uint32_t)buf[i];
-          if (u == v) continue;
-          const uint64_t key = pair_key(u, v);
-          const uint32_t sh = shard_of(key);
-          if ((int)(sh % (uint32_t)nt) != t) continue;
-          fl[i] = shards[sh].insert(key) ? 1 : 0;
+      for (uint32_t sh = (uint32_t)t; sh < nshards; sh += (uint32_t)nt)
+        for (int b = 0; b < threads; b++) {
+          const uint64_t* buf = bufs[b].data();
+          uint8_t* fl = flags.data() + (uint64_t)b * chunk;
+          for (uint32_t i : byshard[b][sh]) {
+            const uint32_t u = (uint32_t)(buf[i] >> 32), v = (uint32_t)buf[i];
+            fl[i] = shards[sh].insert(pair_key(u, v)) ? 1 : 0;
+          }
         }
-      }
     };
     {
       std::vector<std::thread> mk;
@@ -260,7 +272,17 @@ EdgeList gen_rmat(uint32_t scale, uint64_t m, double a, double b, double c, uint
     }
   }
   std::vector<uint32_t> perm = shuffled_ids(n, seed);
-  for (auto& e : edges) e = {perm[e.first], perm[e.second]};
+  {
+    std::vector<std::thread> pm;
+    const uint64_t ne = edges.size();
+    const int tp = ne >= (1u << 20) ? threads : 1;
+    for (int t = 0; t < tp; t++)
+      pm.emplace_back([&, t] {
+        for (uint64_t i = ne * t / tp; i < ne * (t + 1) / tp; i++)
+          edges[i] = {perm[edges[i].first], perm[edges[i].second]};
+      });
+    for (auto& th : pm) th.join();
+  }
   ensure_last_vertex_has_edge(n, edges);
   return edges;
 }
