@@ -23,6 +23,7 @@ const char* io_status_string(int s) {
     case kIoBadRecord: return "Error: truncated or malformed record; undirected survey file";
     case kIoMissingVertex: return "Error: a vertex id has no v line; undirected survey file";
     case kIoEdgeOutOfRange: return "Error: an edge endpoint has no v line; undirected survey file";
+    case kIoWriteFailed: return "Error: the output file could not be written";
   }
   return "Error: unknown";
 }
@@ -359,7 +360,7 @@ int write_embeddings(const std::string& path, uint32_t k, uint64_t count, std::v
   }
   rows.swap(sorted);
   if (fclose(f) != 0) ok = false;
-  return ok ? kIoOk : kIoCannotOpen;
+  return ok ? kIoOk : kIoWriteFailed;
 }
 
 }  // namespace fsig

@@ -17,6 +17,7 @@ enum IoStatus {
   kIoBadRecord = -3,       // truncated or non-numeric record
   kIoMissingVertex = -4,   // an id below the vertex count has no `v` line
   kIoEdgeOutOfRange = -5,  // an edge endpoint has no `v` line
+  kIoWriteFailed = -6,     // an output file could not be written in full
 };
 
 const char* io_status_string(int status);

@@ -78,7 +78,8 @@ const int32_t* fsig_pcsr_labels(const fsig_hpcsr* p);
 
 uint32_t fsig_murmur2(const void* key, int len, uint32_t seed);
 
-/* Canonical embedding file: `#embedding: <count>` then sorted rows. */
+/* Canonical embedding file: `#embedding: <count>` then the rows sorted
+ * lexicographically, one per line.  0, -1 (cannot open) or -6 (write failed). */
 int fsig_write_embeddings(const char* path, uint32_t k, uint64_t count, const uint32_t* rows, uint64_t nrows);
 
 #ifdef __cplusplus

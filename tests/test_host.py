@@ -292,3 +292,13 @@ def test_embedding_file_widths(tmp_path, k):
     assert rc == 0
     want = "#embedding: 5000\n" + "".join(" ".join(map(str, r)) + "\n" for r in sorted(map(tuple, rows.tolist())))
     assert open(path).read() == want
+
+
+def test_embedding_file_errors(tmp_path):
+    import ctypes as C
+    rows = np.zeros((4, 3), np.uint32)
+    ptr = rows.ctypes.data_as(C.POINTER(C.c_uint32))
+    # a directory cannot be opened for writing
+    assert F.host_lib().fsig_write_embeddings(str(tmp_path).encode(), 3, 4, ptr, 4) == -1
+    if os.path.exists("/dev/full"):  # every write to it fails
+        assert F.host_lib().fsig_write_embeddings(b"/dev/full", 3, 4, ptr, 4) == -6
