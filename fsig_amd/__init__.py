@@ -116,18 +116,22 @@ HIP_SYMBOLS = [
     "fsig_set_tune", "fsig_get_tune", "fsig_stream_copy", "fsig_match_batch", "fsig_last_batch_tune",
     "fsig_set_work_limit", "fsig_server_create", "fsig_server_submit", "fsig_server_wait",
     "fsig_server_get_stats", "fsig_server_projected", "fsig_server_last_error", "fsig_server_destroy",
-    "fsig_project_bytes", "fsig_graph_label_pairs",
+    "fsig_project_bytes", "fsig_graph_label_pairs", "fsig_match_digest", "fsig_set_keep_candidates",
+    "fsig_last_candidate_set", "fsig_match_batch_bounded", "fsig_server_submit_bounded",
+    "fsig_server_set_work_limit",
 ]
 
 
 class Tune(C.Structure):
     _fields_ = [("lanes_per_row", C.c_uint32), ("strategy", C.c_uint32), ("lds_tile", C.c_uint32),
-                ("refine_passes", C.c_uint32), ("fused_tail", C.c_uint32), ("nlf_filter", C.c_uint32)]
+                ("refine_passes", C.c_uint32), ("fused_tail", C.c_uint32), ("nlf_filter", C.c_uint32),
+                ("spill_log2", C.c_uint32)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("batches", C.c_uint64),
-                ("fallbacks", C.c_uint64), ("max_batch_seen", C.c_uint64), ("max_concurrent", C.c_uint32),
+                ("fallbacks", C.c_uint64), ("max_batch_seen", C.c_uint64), ("refused", C.c_uint64),
+                ("max_concurrent", C.c_uint32),
                 ("pad", C.c_uint32), ("busy_ms", C.c_double), ("wall_ms", C.c_double)]
 
 
@@ -160,6 +164,9 @@ def hip_lib():
         L.fsig_get_tune.argtypes = [_vp, C.POINTER(Tune)]
         L.fsig_stream_copy.argtypes = [_vp, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]
         L.fsig_set_work_limit.argtypes = [_vp, C.c_uint64]
+        L.fsig_match_digest.argtypes = [_vp, _vp, C.c_uint32, _i32p, _u32p, _u32p, _u32p, C.c_int, _u64p, _u64p]
+        L.fsig_set_keep_candidates.argtypes = [_vp, C.c_int]
+        L.fsig_last_candidate_set.argtypes = [_vp, C.c_uint32, C.c_int, _u32p, C.c_uint32, _u32p]
         _optional_bindings(L)
         _hip = L
     return _hip
@@ -169,10 +176,14 @@ def _optional_bindings(L):
     """Batched and served entry points (see include/fsig.h)."""
     L.fsig_match_batch.argtypes = [_vp, _vp, C.c_uint32, _u32p, C.POINTER(_i32p), C.POINTER(_u32p),
                                    C.POINTER(_u32p), _u64p, C.POINTER(C.c_double)]
+    L.fsig_match_batch_bounded.argtypes = [_vp, _vp, C.c_uint32, _u32p, C.POINTER(_i32p), C.POINTER(_u32p),
+                                           C.POINTER(_u32p), _u64p, _u64p, _i32p, C.POINTER(C.c_double)]
     L.fsig_last_batch_tune.argtypes = [_vp, _u32p, _u32p, _u32p]
     L.fsig_server_create.argtypes = [C.c_int, _vp, C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(Tune),
                                      C.POINTER(_vp)]
     L.fsig_server_submit.argtypes = [_vp, C.c_uint32, _i32p, _u32p, _u32p, _u64p]
+    L.fsig_server_submit_bounded.argtypes = [_vp, C.c_uint32, _i32p, _u32p, _u32p, C.c_uint64, _u64p]
+    L.fsig_server_set_work_limit.argtypes = [_vp, C.c_uint64]
     L.fsig_server_wait.argtypes = [_vp, C.c_uint64, C.c_int64, _u64p, C.POINTER(C.c_double)]
     L.fsig_server_get_stats.argtypes = [_vp, C.POINTER(ServerStats)]
     L.fsig_server_projected.argtypes = [_vp, C.c_uint32, _i32p, _u32p, _u32p, _u64p]
@@ -432,8 +443,9 @@ class Context:
         self.check(self._L.fsig_kernel_stats(self._h, family, C.byref(la), C.byref(ms), C.byref(by), C.byref(un)))
         return {"launches": la.value, "ms": ms.value, "alg_bytes": by.value, "units": un.value}
 
-    def set_tune(self, lanes_per_row=0, strategy=0, lds_tile=0, refine_passes=0, fused_tail=0, nlf_filter=0):
-        t = Tune(lanes_per_row, strategy, lds_tile, refine_passes, fused_tail, nlf_filter)
+    def set_tune(self, lanes_per_row=0, strategy=0, lds_tile=0, refine_passes=0, fused_tail=0, nlf_filter=0,
+                 spill_log2=0):
+        t = Tune(lanes_per_row, strategy, lds_tile, refine_passes, fused_tail, nlf_filter, spill_log2)
         self.check(self._L.fsig_set_tune(self._h, C.byref(t)))
 
     def get_tune(self) -> dict:
@@ -444,6 +456,20 @@ class Context:
     def last_candidates(self, k: int) -> np.ndarray:
         out = np.zeros(k, np.uint32)
         self.check(self._L.fsig_last_candidates(self._h, k, _ptr(out)))
+        return out
+
+    def keep_candidates(self, on: bool = True):
+        """Keep host copies of every match's candidate sets (filter and
+        refinement stages) for last_candidate_set."""
+        self.check(self._L.fsig_set_keep_candidates(self._h, int(on)))
+
+    def last_candidate_set(self, u: int, stage: int = 1) -> np.ndarray:
+        """Ascending candidate vertices of query vertex u in the last match
+        (stage 0: after the filter, 1: after refinement)."""
+        size = C.c_uint32()
+        self.check(self._L.fsig_last_candidate_set(self._h, u, stage, _ptr(None), 0, C.byref(size)))
+        out = np.empty(size.value, np.uint32)
+        self.check(self._L.fsig_last_candidate_set(self._h, u, stage, _ptr(out), size.value, C.byref(size)))
         return out
 
     def set_work_limit(self, max_rows: int):
@@ -527,9 +553,24 @@ def match(dg: DeviceGraph, q: Query, alg: int = ALG_MMATCH, mode: int = MODE_COU
     return res
 
 
-def match_batch(dg: "DeviceGraph", queries: Sequence[Query]) -> Tuple[np.ndarray, float]:
-    """fsig_match_batch: counts of every query (alg 1 semantics) and the
-    persistent join's device time in ms."""
+def match_digest(dg: DeviceGraph, q: Query, alg: int = ALG_MMATCH) -> Tuple[int, Tuple[int, int]]:
+    """fsig_match_digest: (count, order-independent digest of the full
+    embedding set), every embedding enumerated on the device."""
+    ctx = dg.ctx
+    off, adj = q.csr()
+    sig = q.signature() if alg == ALG_GSI else None
+    cnt = C.c_uint64(0)
+    dig = np.zeros(2, np.uint64)
+    ctx.check(ctx._L.fsig_match_digest(ctx._h, dg._h, q.k, _ptr(q.labels, _i32p), _ptr(off), _ptr(adj), _ptr(sig),
+                                       alg, C.byref(cnt), _ptr(dig, _u64p)))
+    return cnt.value, (int(dig[0]), int(dig[1]))
+
+
+def match_batch(dg: "DeviceGraph", queries: Sequence[Query], bounds=None, status: bool = False):
+    """fsig_match_batch (bounds None) or fsig_match_batch_bounded: counts of
+    every query (alg 1 semantics; with bounds[i] != 0, min(count,
+    bounds[i])) and the join's device time in ms; with status, also each
+    query's status (FSIG_ELIMIT: refused by the context's work limit)."""
     ctx = dg.ctx
     nq = len(queries)
     ks = np.array([q.k for q in queries], np.uint32)
@@ -540,8 +581,17 @@ def match_batch(dg: "DeviceGraph", queries: Sequence[Query]) -> Tuple[np.ndarray
     ap = (_u32p * nq)(*[_ptr(a if len(a) else np.zeros(1, np.uint32)) for _, a in csrs])
     counts = np.zeros(nq, np.uint64)
     jm = C.c_double(0)
-    ctx.check(ctx._L.fsig_match_batch(ctx._h, dg._h, nq, _ptr(ks), lp, op, ap, _ptr(counts, _u64p), C.byref(jm)))
-    return counts, jm.value
+    st = np.zeros(nq, np.int32)
+    if bounds is None:
+        ctx.check(ctx._L.fsig_match_batch(ctx._h, dg._h, nq, _ptr(ks), lp, op, ap, _ptr(counts, _u64p),
+                                          C.byref(jm)))
+    else:
+        b = np.ascontiguousarray(np.broadcast_to(np.asarray(bounds, dtype=np.uint64), (nq,)))
+        ctx.check(ctx._L.fsig_match_batch_bounded(ctx._h, dg._h, nq, _ptr(ks), lp, op, ap, _ptr(b, _u64p),
+                                                  _ptr(counts, _u64p), _ptr(st, _i32p), C.byref(jm)))
+        if not status and np.any(st != 0):
+            raise FsigError(int(st[st != 0][0]), "a batch query exceeded the work limit")
+    return (counts, jm.value, st) if status else (counts, jm.value)
 
 
 BATCH_ENGINES = {1: "lockstep", 2: "persistent"}
@@ -597,10 +647,17 @@ class Server:
             raise FsigError(rc, "fsig_server_create failed")
         self._h = h
 
-    def submit(self, q: "Query") -> int:
+    def set_work_limit(self, max_rows: int):
+        rc = self._L.fsig_server_set_work_limit(self._h, max_rows)
+        if rc:
+            raise FsigError(rc, "fsig_server_set_work_limit failed")
+
+    def submit(self, q: "Query", bound: int = 0) -> int:
+        """Queues a count query (bound != 0: min(count, bound)); returns its ticket."""
         lab, off, adj = _query_arrays(q)
         t = C.c_uint64()
-        rc = self._L.fsig_server_submit(self._h, q.k, _ptr(lab, _i32p), _ptr(off), _ptr(adj), C.byref(t))
+        rc = self._L.fsig_server_submit_bounded(self._h, q.k, _ptr(lab, _i32p), _ptr(off), _ptr(adj), bound,
+                                                C.byref(t))
         if rc:
             raise FsigError(rc, self._L.fsig_server_last_error(self._h).decode())
         return t.value

@@ -131,6 +131,37 @@ __global__ void k_emit(EmitMap M, const uint32_t* __restrict__ rows, uint64_t n,
   }
 }
 
+// Digest of level-k rows: one row per thread, words taken in query-vertex
+// order (join position M.pos[i] holds query vertex i); per-wave sums are
+// added to acc[0..1].
+struct DigestMap {
+  uint32_t k;
+  uint32_t pos[MAXQ];             // join position of query vertex i
+  const uint32_t* cand[MAXQ];     // candidate array of join position c (null: word is a vertex id)
+};
+
+__global__ void __launch_bounds__(FSIG_BLOCK) k_digest(DigestMap M, const uint32_t* __restrict__ rows, uint64_t n,
+                                                       unsigned long long* __restrict__ acc) {
+  unsigned long long s0 = 0, s1 = 0;
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t* row = rows + r * M.k;
+    uint64_t x = row_hash_start(M.k);
+    for (uint32_t i = 0; i < M.k; i++) {
+      const uint32_t c = M.pos[i];
+      const uint32_t w = row[c];
+      x = row_hash_step(x, i, M.cand[c] ? M.cand[c][w] : w);
+    }
+    s0 += x;
+    s1 += mix64(x ^ kDigestSalt);
+  }
+  s0 = wave_sum_u64(s0);
+  s1 = wave_sum_u64(s1);
+  if ((threadIdx.x & 63u) == 0) {
+    atomicAdd(&acc[0], s0);
+    atomicAdd(&acc[1], s1);
+  }
+}
+
 uint32_t grid_for(const fsig_ctx* c, uint64_t items, uint32_t per_block) {
   uint64_t b = (items + per_block - 1) / per_block;
   b = std::min<uint64_t>(b, (uint64_t)c->num_cus * 16);
@@ -159,5 +190,66 @@ struct HostSink : LeafSink {
     return FSIG_OK;
   }
 };
+
+// Folds level-k tables into the set digest (acc: two device words, zeroed
+// by the caller before the match).
+struct DigestSink : LeafSink {
+  fsig_ctx* c;
+  DigestMap map;
+  unsigned long long* acc;
+  int take(const uint32_t* rows, uint64_t n) override {
+    if (n == 0) return FSIG_OK;
+    k_digest<<<grid_for(c, n, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(map, rows, n, acc);
+    FSIG_HIP(c, hipGetLastError());
+    return FSIG_OK;
+  }
+};
+
+// The sink a match hands its level-k tables to: the digest sink when the
+// caller asked for a digest, else the host copy when it gave a buffer.
+struct LeafSinks {
+  HostSink host;
+  DigestSink digest;
+  // pos_of_qv[i]: join position of query vertex i; cand_of_pos[c]: candidate
+  // array of position c (null when rows hold vertex ids)
+  LeafSink* setup(fsig_ctx* c, uint32_t k, const uint32_t* order, const uint32_t* const* cand_of_pos,
+                  uint32_t* emb_out, uint64_t emb_cap, unsigned long long* digest_acc) {
+    host.c = c;
+    std::memset(&host.map, 0, sizeof(host.map));
+    host.map.k = k;
+    digest.c = c;
+    std::memset(&digest.map, 0, sizeof(digest.map));
+    digest.map.k = k;
+    digest.acc = digest_acc;
+    for (uint32_t p = 0; p < k; p++) {
+      host.map.qv[p] = order[p];
+      host.map.cand[p] = cand_of_pos ? cand_of_pos[p] : nullptr;
+      digest.map.pos[order[p]] = p;
+      digest.map.cand[p] = cand_of_pos ? cand_of_pos[p] : nullptr;
+    }
+    host.host = emb_out;
+    host.cap = emb_out ? emb_cap : 0;
+    if (digest_acc) return &digest;
+    return emb_out ? &host : nullptr;
+  }
+};
+
+// Host copy of the candidate sets of a match (fsig_set_keep_candidates):
+// stage 0 after the filter, stage 1 after refinement, per class.
+inline int keep_candidate_sets(fsig_ctx* c, int stage, const QueryPlan& P, uint32_t* const* cand,
+                               const uint32_t* size) {
+  if (!c->keep_cand) return FSIG_OK;
+  std::vector<std::vector<uint32_t>> per_class(P.nclass);
+  for (uint32_t a = 0; a < P.nclass; a++) {
+    per_class[a].resize(size[a]);
+    if (size[a])
+      FSIG_HIP(c, hipMemcpyAsync(per_class[a].data(), cand[a], 4ull * size[a], hipMemcpyDeviceToHost, c->stream));
+  }
+  FSIG_HIP(c, hipStreamSynchronize(c->stream));
+  auto& dst = c->cand_sets[stage];
+  dst.assign(P.k, {});
+  for (uint32_t u = 0; u < P.k; u++) dst[u] = per_class[P.cls[u]];
+  return FSIG_OK;
+}
 
 }  // namespace

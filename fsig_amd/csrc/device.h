@@ -90,3 +90,24 @@ __device__ __forceinline__ uint32_t murmur2_u32(uint32_t key, uint32_t seed) {
   h ^= h >> 15;
   return h;
 }
+
+// Order-independent digest of an embedding set.  A row (data vertex ids in
+// query-vertex order) hashes to x by chaining the splitmix64 finaliser
+// (Steele, Lea and Flood; public domain) over its words; the set's digest is
+// the pair (sum of x, sum of mix64(x ^ kDigestSalt)) modulo 2^64, which does
+// not depend on the order rows are produced in.
+constexpr uint64_t kDigestSalt = 0xd6e8feb86659fd93ull;
+
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__host__ __device__ __forceinline__ uint64_t row_hash_start(uint32_t k) {
+  return 0x9e3779b97f4a7c15ull * (uint64_t)(k + 1);
+}
+
+__host__ __device__ __forceinline__ uint64_t row_hash_step(uint64_t x, uint32_t i, uint32_t v) {
+  return mix64(x + (uint64_t)v + ((uint64_t)i << 32));
+}

@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 
@@ -1286,14 +1287,55 @@ struct StageClock {
 static std::map<std::pair<const fsig_ctx*, uint64_t>, TuneChoice> g_tuned;
 static std::mutex g_tuned_mu;
 
-extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint32_t* ks,
-                                const int32_t* const* qlabels, const uint32_t* const* qoff,
-                                const uint32_t* const* qadj, uint64_t* counts_out, double* join_ms) {
+// Level-p rows of one query beyond which a bounded query leaves the lockstep
+// join for the bounded depth-first driver (fsig_tune.spill_log2 overrides):
+// past this size the chunked driver, which stops at the bound, does less
+// work than a whole level.
+constexpr uint32_t kSpillLog2 = 20;
+
+// Level descriptions of one query of a batch for the single-query driver.
+struct BatchProvider {
+  using SrcType = CandEdgeSrc;
+  std::function<void(uint32_t, JoinLevel&, CandEdgeSrc&)> fill;
+  JoinLevel level(uint32_t p) const {
+    JoinLevel L;
+    CandEdgeSrc S;
+    std::memset(&L, 0, sizeof(L));
+    std::memset(&S, 0, sizeof(S));
+    fill(p, L, S);
+    return L;
+  }
+  CandEdgeSrc src(uint32_t p) const {
+    JoinLevel L;
+    CandEdgeSrc S;
+    std::memset(&L, 0, sizeof(L));
+    std::memset(&S, 0, sizeof(S));
+    fill(p, L, S);
+    return S;
+  }
+};
+
+// bounds (may be null): per query, 0 = exact count, else the count is
+// min(count, bounds[i]) and the search of query i stops once that many
+// embeddings are found (fsig_match's max_embeddings).
+static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint32_t* ks,
+                      const int32_t* const* qlabels, const uint32_t* const* qoff, const uint32_t* const* qadj,
+                      const uint64_t* bounds, uint64_t* counts_out, int32_t* status_out, double* join_ms) {
   if (!c) return FSIG_EINVAL;
   if (!g || !ks || !qlabels || !qoff || !qadj || !counts_out) return set_err(c, FSIG_EINVAL, "batch: null argument");
   if (join_ms) *join_ms = 0;
   if (nq == 0) return FSIG_OK;
   FSIG_HIP(c, hipSetDevice(c->device));
+  std::vector<uint64_t> bound(nq, 0);
+  bool any_bound = false;
+  for (uint32_t i = 0; bounds && i < nq; i++) {
+    bound[i] = bounds[i];
+    any_bound |= bounds[i] != 0;
+  }
+  // results of bounded queries finished outside the lockstep join
+  std::vector<uint8_t> settled(nq, 0);
+  std::vector<uint64_t> settled_count(nq, 0);
+  std::vector<int> settled_rc(nq, FSIG_OK);
   const uint32_t n = g->n;
   Arena& A = c->arena;
   const size_t mark = A.mark();
@@ -1394,11 +1436,14 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
     const QueryPlan& P = plans[i];
     for (uint32_t a = 0; a < P.nclass; a++) {
       const uint32_t gc = cls_base[i] + a;
+      // an inactive query may carry a label the graph does not have: its
+      // segments are empty and the label never indexes the inverted index
       const int32_t lab = P.class_label[a];
       const uint32_t len = active[i] ? g->h_inv_index[lab + 1] - g->h_inv_index[lab] : 0;
+      const uint32_t* src = active[i] ? g->d_inv_list + g->h_inv_index[lab] : g->d_inv_list;
       cand_a[gc] = (uint32_t*)A.alloc(4ull * std::max<uint32_t>(len, 1));
       if (!cand_a[gc]) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted for candidates");
-      comp[gc] = BComp{g->d_inv_list + g->h_inv_index[lab], cand_a[gc], masks + (uint64_t)i * n, len, a, 0, tiles};
+      comp[gc] = BComp{src, cand_a[gc], masks + (uint64_t)i * n, len, a, 0, tiles};
       tiles += (len + kTile - 1) / kTile;
     }
   }
@@ -1807,6 +1852,66 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
     return M;
   };
   unsigned long long* stats_ls = family_counters(c, FSIG_K_BATCH);
+  // Lanes of the single-query driver for query i: from the mean length of
+  // its candidate-edge lists, as fsig_match picks them.
+  auto lanes_for = [&](uint32_t i) {
+    uint64_t edges = 0, rows = 0;
+    for (uint32_t x = 0; x < MAXQ * MAXQ; x++) {
+      const int pi = pair_of[i].empty() ? -1 : pair_of[i][x];
+      if (pi < 0) continue;
+      edges += ptotal_h[pi];
+      rows += pairs[pi].asize;
+    }
+    return pick_lanes(c, rows ? (double)edges / (double)rows : 1.0);
+  };
+  // Bounded queries at a lockstep level, after its count pass and scan: a
+  // query whose count already reached its bound is settled at the bound; a
+  // query whose next level would exceed kSpillRows rows is finished from its
+  // level-p rows by the bounded depth-first driver, which stops at the bound.
+  // Either way its rows get no children here, and the level is rescanned.
+  const uint64_t spill_rows = 1ull << (c->tune.spill_log2 ? c->tune.spill_log2 : kSpillLog2);
+  auto settle_bounded = [&](uint32_t p, const uint32_t* rows, uint32_t* cnt, uint64_t* off, uint64_t R,
+                            const uint64_t* dqb, uint64_t* dqb2, const std::vector<uint64_t>& qb,
+                            const std::vector<uint32_t>& live, uint64_t& T) -> int {
+    const uint32_t nl = (uint32_t)live.size();
+    k_gather_bases<<<(nl + 256) / 256, 256, 0, c->stream>>>(off, dqb, nl, dqb2);
+    FSIG_HIP(c, hipGetLastError());
+    std::vector<uint64_t> nb(nl + 1), hc(nq);
+    int rc2;
+    for (uint32_t j = 0; j <= nl; j += kPinnedWords)
+      if ((rc2 = read_u64(c, dqb2 + j, nb.data() + j, std::min<size_t>(kPinnedWords, nl + 1 - j)))) return rc2;
+    for (uint32_t i = 0; i < nq; i += kPinnedWords)
+      if ((rc2 = read_u64(c, dcounts + i, hc.data() + i, std::min<size_t>(kPinnedWords, nq - i)))) return rc2;
+    bool dropped = false;
+    for (uint32_t j = 0; j < nl; j++) {
+      const uint32_t i = live[j];
+      if (!bound[i] || settled[i] || qb[j + 1] == qb[j]) continue;
+      if (hc[i] >= bound[i]) {
+        settled[i] = 1;
+        settled_count[i] = bound[i];
+      } else if (nb[j + 1] - nb[j] > spill_rows) {
+        BatchProvider pv;
+        pv.fill = [&fill_level, i](uint32_t pp, JoinLevel& L, CandEdgeSrc& S) { fill_level(i, pp, L, S); };
+        JoinDriver<BatchProvider> drv(c, pv, ks[i], lanes_for(i), FSIG_MODE_COUNT, bound[i], nullptr);
+        const size_t dm = A.mark();
+        rc2 = drv.run_at(p, rows + qb[j] * p, qb[j + 1] - qb[j]);
+        A.release(dm);
+        if (rc2 && rc2 != FSIG_ELIMIT) return rc2;
+        settled[i] = 1;
+        settled_count[i] = rc2 ? 0 : drv.total();
+        settled_rc[i] = rc2;  // over the context's work limit: refused
+      } else {
+        continue;
+      }
+      k_fill_u32<<<bgrid(c, qb[j + 1] - qb[j], FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(cnt + qb[j], qb[j + 1] - qb[j],
+                                                                                       0);
+      dropped = true;
+    }
+    if (!dropped) return FSIG_OK;
+    FSIG_HIP(c, hipGetLastError());
+    if ((rc2 = scan_u32_to_u64(c, cnt, off, R))) return rc2;
+    return read_u64(c, off + R, &T, 1);
+  };
   // Lockstep batched BFS.  G = 0: lanes by role at each level: 64 for the
   // queries whose last two levels are counted here (long shared
   // intersections, a 1024-word LDS tile), 16 for the others.  The live
@@ -1898,6 +2003,10 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
         if ((rc2 = scan_u32_to_u64(c, cnt, off, R))) { A.release(mk); return rc2; }
         uint64_t T;
         if ((rc2 = read_u64(c, off + R, &T, 1))) { A.release(mk); return rc2; }
+        if (any_bound && (rc2 = settle_bounded(p, rows, cnt, off, R, dqb, dqb2, qb, live, T))) {
+          A.release(mk);
+          return rc2;
+        }
         if (clk.on) {
           uint64_t st2[2] = {0, 0};
           if (stats_ls && read_u64(c, stats_ls, st2, 2) == FSIG_OK) {
@@ -1987,7 +2096,9 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
     if (active[i]) all_q.push_back(i);
   const uint64_t shape = ((uint64_t)nq << 40) ^ ((uint64_t)g->n << 8) ^ (uint64_t)stack_words;
   TuneChoice best{16, false, 0, true};
-  if (c->tune.strategy == 3) {
+  if (any_bound || c->tune.strategy == 3) {
+    // bounded batches settle queries inside the lockstep join, which timing
+    // runs of the autotuner would repeat
     best = TuneChoice{0, false, 0, true};  // lockstep, lanes per query
   } else if (c->tune.lanes_per_row) {
     best = TuneChoice{c->tune.lanes_per_row, c->tune.strategy == 2, c->tune.lds_tile ? c->tune.lds_tile : 256,
@@ -2044,7 +2155,7 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
     rc = best.lockstep ? lockstep(best.G, all_q, ms) : persistent(best, all_q, true, ms, family_counters(c, FSIG_K_BATCH));
     tj.done(0, 0);
   }
-  if (rc == FSIG_ENOMEM && best.lockstep) {  // a level did not fit: the walk needs no tables
+  if (rc == FSIG_ENOMEM && best.lockstep && !any_bound) {  // a level did not fit: the walk needs no tables
     FSIG_HIP(c, hipMemsetAsync(dcounts, 0, 8ull * nq, c->stream));
     best = TuneChoice{64, false, 0, false};
     rc = persistent(best, all_q, true, ms, family_counters(c, FSIG_K_BATCH));
@@ -2057,7 +2168,13 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
   for (uint32_t i = 0; i < nq; i++) {
     counts_out[i] = active[i] ? hc[i] : 0;
     if (active[i] && ks[i] == 1) counts_out[i] = size[cls_base[i] + plans[i].cls[0]];
+    if (settled[i]) counts_out[i] = settled_count[i];
+    if (bound[i]) counts_out[i] = std::min(counts_out[i], bound[i]);
+    if (status_out) status_out[i] = settled_rc[i];
   }
+  bool refused = false;
+  for (uint32_t i = 0; i < nq; i++) refused |= settled_rc[i] != FSIG_OK;
+  if (refused && !status_out) return set_err(c, FSIG_ELIMIT, "batch: a query exceeded the work limit");
   clk.mark("join");
   if (join_ms) *join_ms = ms;
   c->last_tune_G = best.G;
@@ -2066,4 +2183,18 @@ extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, c
   c->last_peak = A.peak - mark;
   FSIG_HIP(c, hipStreamSynchronize(c->stream));
   return resolve_timings(c);
+}
+
+extern "C" int fsig_match_batch(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint32_t* ks,
+                                const int32_t* const* qlabels, const uint32_t* const* qoff,
+                                const uint32_t* const* qadj, uint64_t* counts_out, double* join_ms) {
+  return batch_impl(c, g, nq, ks, qlabels, qoff, qadj, nullptr, counts_out, nullptr, join_ms);
+}
+
+extern "C" int fsig_match_batch_bounded(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint32_t* ks,
+                                        const int32_t* const* qlabels, const uint32_t* const* qoff,
+                                        const uint32_t* const* qadj, const uint64_t* max_embeddings,
+                                        uint64_t* counts_out, int32_t* status_out, double* join_ms) {
+  if (c && !max_embeddings) return set_err(c, FSIG_EINVAL, "batch: null bounds");
+  return batch_impl(c, g, nq, ks, qlabels, qoff, qadj, max_embeddings, counts_out, status_out, join_ms);
 }

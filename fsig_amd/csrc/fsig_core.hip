@@ -66,6 +66,12 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_scan_tiles(const uint32_t* __res
   }
 }
 
+// Zeroes a few words (a kernel rather than a memset, so it may sit in a
+// captured graph).
+__global__ void k_fill_words(uint32_t* __restrict__ p, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
 // Plain copy, 16 bytes per thread per step (the STREAM "copy" kernel).
 __global__ void __launch_bounds__(FSIG_BLOCK) k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                             uint64_t n) {
@@ -355,9 +361,13 @@ void fsig_graph_free(fsig_ctx* c, fsig_graph* g) {
   delete g;
 }
 
-int fsig_match(fsig_ctx* c, const fsig_graph* g, uint32_t k, const int32_t* qlabels, const uint32_t* qoff,
-               const uint32_t* qadj, const uint32_t* qsig, int alg, int mode, uint64_t max_emb,
-               uint64_t* count_out, uint32_t* emb_out, uint64_t emb_cap) {
+}  // extern "C"
+
+// One match; with digest_out the embeddings are enumerated into the set
+// digest instead of being copied out.
+static int match_impl(fsig_ctx* c, const fsig_graph* g, uint32_t k, const int32_t* qlabels, const uint32_t* qoff,
+                      const uint32_t* qadj, const uint32_t* qsig, int alg, int mode, uint64_t max_emb,
+                      uint64_t* count_out, uint32_t* emb_out, uint64_t emb_cap, uint64_t* digest_out) {
   if (!c) return FSIG_EINVAL;
   if (!g || !qlabels || !qoff || !count_out) return set_err(c, FSIG_EINVAL, "match: null argument");
   if (k > 0 && qoff[k] > 0 && !qadj) return set_err(c, FSIG_EINVAL, "match: null query adjacency");
@@ -365,26 +375,75 @@ int fsig_match(fsig_ctx* c, const fsig_graph* g, uint32_t k, const int32_t* qlab
   if (alg != FSIG_ALG_MMATCH && alg != FSIG_ALG_GSI) return set_err(c, FSIG_EINVAL, "match: bad algorithm");
   if (alg == FSIG_ALG_GSI && !g->has_pcsr) return set_err(c, FSIG_ENOTFOUND, "match: graph has no PCSR upload");
   *count_out = 0;
+  if (digest_out) digest_out[0] = digest_out[1] = 0;
   FSIG_HIP(c, hipSetDevice(c->device));
   QueryPlan P;
   int rc = plan_classes(P, k, qlabels, qoff, qadj, qsig, alg, c->err);
   if (rc) return rc;
+  c->cand_sets[0].clear();
+  c->cand_sets[1].clear();
   for (uint32_t u = 0; u < k; u++)
     if ((uint32_t)qlabels[u] >= g->num_labels) {
       // a label absent from the data graph has no candidates
       c->last_cand.assign(k, 0);
+      if (c->keep_cand) c->cand_sets[0].assign(k, {}), c->cand_sets[1].assign(k, {});
       return FSIG_OK;
     }
   const size_t mk = c->arena.mark();
   c->arena.peak = mk;
-  rc = alg == FSIG_ALG_MMATCH ? run_mmatch(c, g, P, mode, max_emb, count_out, emb_out, emb_cap)
-                              : run_gsi(c, g, P, mode, max_emb, count_out, emb_out, emb_cap);
+  unsigned long long* acc = nullptr;
+  if (digest_out) {
+    acc = (unsigned long long*)c->arena.alloc(2 * sizeof(unsigned long long));
+    if (!acc) return set_err(c, FSIG_ENOMEM, "digest: arena exhausted");
+    k_fill_words<<<1, 64, 0, c->stream>>>((uint32_t*)acc, 4);
+    FSIG_HIP(c, hipGetLastError());
+  }
+  rc = alg == FSIG_ALG_MMATCH ? run_mmatch(c, g, P, mode, max_emb, count_out, emb_out, emb_cap, acc)
+                              : run_gsi(c, g, P, mode, max_emb, count_out, emb_out, emb_cap, acc);
+  if (rc == FSIG_OK && acc) rc = read_u64(c, acc, digest_out, 2);
   c->last_peak = c->arena.peak - mk;
   c->arena.release(mk);
   hipError_t e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return set_err(c, FSIG_EHIP, std::string("match: ") + hipGetErrorString(e));
   int rt = resolve_timings(c);
   return rc ? rc : rt;
+}
+
+extern "C" {
+
+int fsig_match(fsig_ctx* c, const fsig_graph* g, uint32_t k, const int32_t* qlabels, const uint32_t* qoff,
+               const uint32_t* qadj, const uint32_t* qsig, int alg, int mode, uint64_t max_emb,
+               uint64_t* count_out, uint32_t* emb_out, uint64_t emb_cap) {
+  return match_impl(c, g, k, qlabels, qoff, qadj, qsig, alg, mode, max_emb, count_out, emb_out, emb_cap, nullptr);
+}
+
+int fsig_match_digest(fsig_ctx* c, const fsig_graph* g, uint32_t k, const int32_t* qlabels, const uint32_t* qoff,
+                      const uint32_t* qadj, const uint32_t* qsig, int alg, uint64_t* count_out,
+                      uint64_t* digest_out) {
+  if (c && !digest_out) return set_err(c, FSIG_EINVAL, "digest: null digest");
+  return match_impl(c, g, k, qlabels, qoff, qadj, qsig, alg, FSIG_MODE_ENUMERATE, 0, count_out, nullptr, 0,
+                    digest_out);
+}
+
+int fsig_set_keep_candidates(fsig_ctx* c, int on) {
+  if (!c) return FSIG_EINVAL;
+  c->keep_cand = on != 0;
+  if (!c->keep_cand) {
+    c->cand_sets[0].clear();
+    c->cand_sets[1].clear();
+  }
+  return FSIG_OK;
+}
+
+int fsig_last_candidate_set(const fsig_ctx* c, uint32_t u, int stage, uint32_t* out, uint32_t cap,
+                            uint32_t* size) {
+  if (!c || !size || stage < 0 || stage > 1) return FSIG_EINVAL;
+  const auto& sets = c->cand_sets[stage];
+  if (u >= sets.size()) return FSIG_ENOTFOUND;
+  const auto& v = sets[u];
+  *size = (uint32_t)v.size();
+  if (out) std::memcpy(out, v.data(), 4ull * std::min<size_t>(cap, v.size()));
+  return FSIG_OK;
 }
 
 int fsig_stream_copy(fsig_ctx* c, uint64_t bytes, uint32_t iters, double* gbps) {
@@ -463,6 +522,7 @@ int fsig_set_tune(fsig_ctx* c, const fsig_tune* t) {
   if (t->strategy > 3) return set_err(c, FSIG_EINVAL, "tune: strategy");
   if (t->fused_tail > 1) return set_err(c, FSIG_EINVAL, "tune: fused_tail");
   if (t->nlf_filter > 1) return set_err(c, FSIG_EINVAL, "tune: nlf_filter");
+  if (t->spill_log2 > 40) return set_err(c, FSIG_EINVAL, "tune: spill_log2");
   c->tune = *t;
   return FSIG_OK;
 }

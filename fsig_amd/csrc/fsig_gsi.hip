@@ -107,7 +107,7 @@ struct GsiProvider {
 }  // namespace
 
 int run_gsi(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_t max_emb, uint64_t* count_out,
-            uint32_t* emb_out, uint64_t emb_cap) {
+            uint32_t* emb_out, uint64_t emb_cap, unsigned long long* digest_acc) {
   const uint32_t nc = P.nclass, k = P.k;
   Arena& A = c->arena;
   int rc;
@@ -176,6 +176,9 @@ int run_gsi(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_t m
   tf.done(4ull * g->n + items * (4 + 60 + 4) + 8ull * items * nc + 4ull * cand_total, items);
   c->last_cand.assign(k, 0);
   for (uint32_t u = 0; u < k; u++) c->last_cand[u] = size[P.cls[u]];
+  // the signature filter is alg 2's only filter: both stages hold its sets
+  if ((rc = keep_candidate_sets(c, 0, P, cand, size))) return rc;
+  if ((rc = keep_candidate_sets(c, 1, P, cand, size))) return rc;
   if (empty) { *count_out = 0; return FSIG_OK; }
 
   plan_order_gsi(P, size);
@@ -186,14 +189,9 @@ int run_gsi(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_t m
   for (uint32_t a = 0; a < nc; a++) { pv.cand[a] = cand[a]; pv.csize[a] = size[a]; }
   const double avg = g->n ? (double)g->nnz / g->n / std::max<uint32_t>(g->num_labels, 1) : 1;
   const uint32_t G = pick_lanes(c, avg);
-  HostSink sink;
-  sink.c = c;
-  std::memset(&sink.map, 0, sizeof(sink.map));
-  sink.map.k = k;
-  for (uint32_t pos = 0; pos < k; pos++) sink.map.qv[pos] = P.order[pos];
-  sink.host = emb_out;
-  sink.cap = emb_out ? emb_cap : 0;
-  JoinDriver<GsiProvider> drv(c, pv, k, G, mode, max_emb, (mode == FSIG_MODE_ENUMERATE && emb_out) ? &sink : nullptr);
+  LeafSinks sinks;  // rows hold data vertex ids
+  LeafSink* sink = sinks.setup(c, k, P.order, nullptr, emb_out, emb_cap, digest_acc);
+  JoinDriver<GsiProvider> drv(c, pv, k, G, mode, max_emb, mode == FSIG_MODE_ENUMERATE ? sink : nullptr);
   // level-1 rows are the candidate vertices of the first join vertex
   if ((rc = drv.run(cand[P.cls[P.order[0]]], size[P.cls[P.order[0]]]))) return rc;
   *count_out = drv.total();

@@ -372,7 +372,7 @@ static uint32_t refine_lanes(const fsig_graph* g) {
 }
 
 int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_t max_emb, uint64_t* count_out,
-               uint32_t* emb_out, uint64_t emb_cap) {
+               uint32_t* emb_out, uint64_t emb_cap, unsigned long long* digest_acc) {
   const uint32_t nc = P.nclass, k = P.k;
   Arena& A = c->arena;
   int rc;
@@ -443,6 +443,7 @@ int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_
   if ((rc = read_u64(c, dsizes, hs, nc))) return rc;
   uint64_t cand_total = 0;
   for (uint32_t a = 0; a < nc; a++) { size[a] = (uint32_t)hs[a]; cand_total += size[a]; }
+  if ((rc = keep_candidate_sets(c, 0, P, cand_a, size))) return rc;
   tf.done(4ull * g->n + 12ull * ldf_items + 4ull * ldf_items * nc + 4ull * cand_total, ldf_items);
 
   // ---- refinement passes
@@ -516,6 +517,7 @@ int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_
   }
   c->last_cand.assign(k, 0);
   for (uint32_t u = 0; u < k; u++) c->last_cand[u] = size[P.cls[u]];
+  if ((rc = keep_candidate_sets(c, 1, P, cur, size))) return rc;
   if (empty) { *count_out = 0; return FSIG_OK; }
 
   // ---- join order and the class pairs it needs
@@ -640,18 +642,11 @@ int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_
   uint32_t* level1 = (uint32_t*)A.alloc(4ull * std::max<uint32_t>(n1, 1));
   if (!level1) return set_err(c, FSIG_ENOMEM, "join: arena exhausted");
   k_iota<<<grid_for(c, n1, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(level1, n1);
-  HostSink sink;
-  sink.c = c;
-  std::memset(&sink.map, 0, sizeof(sink.map));
-  sink.map.k = k;
-  for (uint32_t pos = 0; pos < k; pos++) {
-    sink.map.qv[pos] = P.order[pos];
-    sink.map.cand[pos] = cur[P.cls[P.order[pos]]];
-  }
-  sink.host = emb_out;
-  sink.cap = emb_out ? emb_cap : 0;
-  JoinDriver<MMatchProvider> drv(c, pv, k, G, mode, max_emb,
-                                 (mode == FSIG_MODE_ENUMERATE && emb_out) ? &sink : nullptr);
+  const uint32_t* cand_of_pos[MAXQ];
+  for (uint32_t pos = 0; pos < k; pos++) cand_of_pos[pos] = cur[P.cls[P.order[pos]]];
+  LeafSinks sinks;
+  LeafSink* sink = sinks.setup(c, k, P.order, cand_of_pos, emb_out, emb_cap, digest_acc);
+  JoinDriver<MMatchProvider> drv(c, pv, k, G, mode, max_emb, mode == FSIG_MODE_ENUMERATE ? sink : nullptr);
   if ((rc = drv.run(level1, n1))) return rc;
   *count_out = drv.total();
   return FSIG_OK;

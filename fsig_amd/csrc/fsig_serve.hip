@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_label_pairs(const uint32_t* __re
 // counts (the first backward neighbour contributes the mean number of
 // label-matching neighbours, each further one the probability of an edge).
 static uint64_t project_bytes(uint32_t n, uint32_t L, const uint64_t* lcount, const uint64_t* pairs,
-                              const QueryPlan& P) {
+                              const QueryPlan& P, bool bounded = false) {
   const uint32_t k = P.k;
   for (uint32_t u = 0; u < k; u++)
     if (P.label[u] < 0 || (uint32_t)P.label[u] >= L) return 4096;  // label absent: the query is dead
@@ -110,6 +110,9 @@ static uint64_t project_bytes(uint32_t n, uint32_t L, const uint64_t* lcount, co
       first = false;
     }
     rows = std::min(rows * f, 1e15);
+    // a bounded query leaves the lockstep join before a level passes 2^20
+    // rows (the depth-first driver then keeps windows of the free arena)
+    if (bounded) rows = std::min(rows, (double)(1u << 20));
     if (p + 1 == k) break;  // the last level is counted, not stored
   }
   const double total = (double)bytes + level_bytes;
@@ -157,6 +160,7 @@ extern "C" int fsig_graph_label_pairs(fsig_ctx* c, const fsig_graph* g, uint64_t
 struct Request {
   uint64_t ticket;
   uint32_t k;
+  uint64_t bound;  // 0: exact count, else min(count, bound)
   std::vector<int32_t> labels;
   std::vector<uint32_t> off, adj;
   uint64_t projected;
@@ -223,21 +227,27 @@ static void worker_loop(fsig_server* s, uint32_t wi) {
       qo[i] = batch[i].off.data();
       qa[i] = batch[i].adj.data();
     }
-    std::vector<uint64_t> counts(nb, 0);
+    std::vector<uint64_t> counts(nb, 0), bounds(nb, 0);
+    for (uint32_t i = 0; i < nb; i++) bounds[i] = batch[i].bound;
+    std::vector<int32_t> st(nb, FSIG_OK);
     std::vector<int> rcs(nb, FSIG_OK);
-    int rc = fsig_match_batch(c, s->g, nb, ks.data(), ql.data(), qo.data(), qa.data(), counts.data(), nullptr);
-    uint64_t fallbacks = 0;
+    int rc = fsig_match_batch_bounded(c, s->g, nb, ks.data(), ql.data(), qo.data(), qa.data(), bounds.data(),
+                                      counts.data(), st.data(), nullptr);
+    uint64_t fallbacks = 0, refused = 0;
     if (rc == FSIG_ENOMEM) {
       // the projection was short: each query alone through the plain path,
       // whose join is memory-blocked
       for (uint32_t i = 0; i < nb; i++) {
-        rcs[i] = fsig_match(c, s->g, ks[i], ql[i], qo[i], qa[i], nullptr, FSIG_ALG_MMATCH, FSIG_MODE_COUNT, 0,
-                            &counts[i], nullptr, 0);
+        rcs[i] = fsig_match(c, s->g, ks[i], ql[i], qo[i], qa[i], nullptr, FSIG_ALG_MMATCH, FSIG_MODE_COUNT,
+                            bounds[i], &counts[i], nullptr, 0);
         fallbacks++;
       }
     } else if (rc) {
       for (uint32_t i = 0; i < nb; i++) rcs[i] = rc;
+    } else {
+      for (uint32_t i = 0; i < nb; i++) rcs[i] = st[i];
     }
+    for (uint32_t i = 0; i < nb; i++) refused += rcs[i] == FSIG_ELIMIT;
     const auto t1 = std::chrono::steady_clock::now();
     {
       std::lock_guard<std::mutex> lk(s->mu);
@@ -251,6 +261,7 @@ static void worker_loop(fsig_server* s, uint32_t wi) {
       s->running--;
       s->st.completed += nb;
       s->st.fallbacks += fallbacks;
+      s->st.refused += refused;
       s->st.busy_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
       s->t_last = t1;
     }
@@ -276,7 +287,7 @@ extern "C" int fsig_server_create(int device, const fsig_graph* g, uint32_t work
   // a batch is admitted against 3/4 of its worker's arena: the rest covers
   // the tables and scan temporaries the projection leaves out
   s->share = per / 4 * 3;
-  fsig_tune t{0, 3, 0, 0, 0, 0};  // lockstep join, lanes per query, no autotuning
+  fsig_tune t{0, 3, 0, 0, 0, 0, 0};  // lockstep join, lanes per query, no autotuning
   if (tune) t = *tune;
   for (uint32_t w = 0; w < workers; w++) {
     fsig_ctx* c = nullptr;
@@ -307,6 +318,11 @@ extern "C" int fsig_server_create(int device, const fsig_graph* g, uint32_t work
 
 extern "C" int fsig_server_submit(fsig_server* s, uint32_t k, const int32_t* qlabels, const uint32_t* qoff,
                                   const uint32_t* qadj, uint64_t* ticket) {
+  return fsig_server_submit_bounded(s, k, qlabels, qoff, qadj, 0, ticket);
+}
+
+extern "C" int fsig_server_submit_bounded(fsig_server* s, uint32_t k, const int32_t* qlabels, const uint32_t* qoff,
+                                          const uint32_t* qadj, uint64_t max_embeddings, uint64_t* ticket) {
   if (!s || !qlabels || !qoff || !qadj || !ticket) return FSIG_EINVAL;
   QueryPlan P;
   std::string err;
@@ -318,10 +334,11 @@ extern "C" int fsig_server_submit(fsig_server* s, uint32_t k, const int32_t* qla
   }
   Request r;
   r.k = k;
+  r.bound = max_embeddings;
   r.labels.assign(qlabels, qlabels + k);
   r.off.assign(qoff, qoff + k + 1);
   r.adj.assign(qadj, qadj + qoff[k]);
-  r.projected = project_bytes(s->g->n, s->g->num_labels, s->lcount.data(), s->pairs.data(), P);
+  r.projected = project_bytes(s->g->n, s->g->num_labels, s->lcount.data(), s->pairs.data(), P, max_embeddings != 0);
   r.t_submit = std::chrono::steady_clock::now();
   {
     std::lock_guard<std::mutex> lk(s->mu);
@@ -359,6 +376,13 @@ extern "C" int fsig_server_wait(fsig_server* s, uint64_t ticket, int64_t timeout
   if (count) *count = r.count;
   if (latency_ms) *latency_ms = r.latency_ms;
   return r.rc;
+}
+
+extern "C" int fsig_server_set_work_limit(fsig_server* s, uint64_t max_rows) {
+  if (!s) return FSIG_EINVAL;
+  // workers read the limit when a batch starts; set it before submitting
+  for (fsig_ctx* c : s->workers) fsig_set_work_limit(c, max_rows);
+  return FSIG_OK;
 }
 
 extern "C" int fsig_server_get_stats(fsig_server* s, fsig_server_stats* out) {

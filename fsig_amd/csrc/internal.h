@@ -64,8 +64,12 @@ struct fsig_ctx {
   // resolved after the stream drains (no synchronisation per launch).
   std::vector<TimedLaunch> timed;
   size_t timed_used = 0;
-  fsig_tune tune{0, 0, 0, 0, 0, 0};
+  fsig_tune tune{0, 0, 0, 0, 0, 0, 0};
   std::vector<uint32_t> last_cand;
+  // candidate sets of the last fsig_match per query vertex, kept when
+  // keep_cand is set: [0] after the filter, [1] after refinement
+  bool keep_cand = false;
+  std::vector<std::vector<uint32_t>> cand_sets[2];
   uint64_t last_peak = 0;
   // device counters (64 words): [0..7] per-launch results, [16 + 4f ..] per
   // kernel family f accumulated device-side statistics ([0] algorithmic
@@ -197,7 +201,10 @@ int scan_u32_to_u64(fsig_ctx* c, const uint32_t* in, uint64_t* out, uint64_t n);
 int read_u64(fsig_ctx* c, const void* dev, uint64_t* host, size_t count);
 
 // The per-query matcher pipelines.
+// digest_acc (two zeroed device words, may be null): enumerate every
+// embedding and fold it into the order-independent set digest there instead
+// of copying rows to emb_out.
 int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_t max_emb,
-               uint64_t* count_out, uint32_t* emb_out, uint64_t emb_cap);
+               uint64_t* count_out, uint32_t* emb_out, uint64_t emb_cap, unsigned long long* digest_acc);
 int run_gsi(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_t max_emb,
-            uint64_t* count_out, uint32_t* emb_out, uint64_t emb_cap);
+            uint64_t* count_out, uint32_t* emb_out, uint64_t emb_cap, unsigned long long* digest_acc);

@@ -672,6 +672,8 @@ class JoinDriver {
   uint64_t total() const { return total_; }
 
   int run(const uint32_t* level1, uint64_t n1) { return expand(1, level1, n1); }
+  // Continues from R level-p rows (p words each) produced elsewhere.
+  int run_at(uint32_t p, const uint32_t* rows, uint64_t R) { return expand(p, rows, R); }
 
  private:
   bool bounded() const { return max_emb_ != 0; }

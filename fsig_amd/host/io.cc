@@ -259,9 +259,10 @@ int write_graph(const std::string& path, Vertex n, const Label* labels, const Ed
   std::vector<char> out;
   out.reserve(1 << 20);
   char line[96];
+  bool short_write = false;
   auto flush = [&](bool force) {
     if (force || out.size() > (1 << 20)) {
-      fwrite(out.data(), 1, out.size(), f);
+      if (!out.empty() && fwrite(out.data(), 1, out.size(), f) != out.size()) short_write = true;
       out.clear();
     }
   };
@@ -279,8 +280,10 @@ int write_graph(const std::string& path, Vertex n, const Label* labels, const Ed
     for (Edge i = off[v]; i < off[v + 1]; i++)
       if (adj[i] > v) emit(snprintf(line, sizeof line, "e %u %u", v, adj[i]));
   flush(true);
-  fclose(f);
-  return kIoOk;
+  // a short write or a failed close (buffered data that never reached the
+  // file) leaves a truncated graph: report it
+  if (fclose(f) != 0) short_write = true;
+  return short_write ? kIoWriteFailed : kIoOk;
 }
 
 // Rows are sorted lexicographically (an index sort: chunks on threads, then

@@ -101,6 +101,36 @@ int fsig_match(fsig_ctx* ctx, const fsig_graph* g, uint32_t k, const int32_t* ql
                const uint32_t* qoff, const uint32_t* qadj, const uint32_t* qsig, int alg, int mode,
                uint64_t max_embeddings, uint64_t* count_out, uint32_t* emb_out, uint64_t emb_cap);
 
+/* Enumerates every embedding of one query (as fsig_match with
+ * FSIG_MODE_ENUMERATE, windowed through the arena, nothing copied to the
+ * host) and folds each into an order-independent digest of the set: a row
+ * r = (r_0 .. r_{k-1}) of data vertex ids in query-vertex order hashes to
+ *   x = fold_{i=0..k-1} x <- mix64(x + r_i + (i << 32)),  x_0 = 0x9e3779b97f4a7c15 * (k + 1)
+ * with mix64 the splitmix64 finaliser, and the set's digest is
+ *   digest_out[0] = sum of x,  digest_out[1] = sum of mix64(x ^ 0xd6e8feb86659fd93)
+ * modulo 2^64.  Two sets with equal digests and counts are equal up to a
+ * 2^-64-scale collision; the digest does not depend on the order rows are
+ * found in, so it compares engines and join orders.  *count_out is the
+ * number of embeddings.  No counterpart in the reference, whose output is
+ * the count and a few rows (match/MMatch.cu:2046-2074, match/GSI.cu:310-326). */
+int fsig_match_digest(fsig_ctx* ctx, const fsig_graph* g, uint32_t k, const int32_t* qlabels,
+                      const uint32_t* qoff, const uint32_t* qadj, const uint32_t* qsig, int alg,
+                      uint64_t* count_out, uint64_t* digest_out);
+
+/* Candidate sets of the last fsig_match / fsig_match_digest of a context,
+ * kept on the host while fsig_set_keep_candidates(ctx, 1) is in force (a
+ * copy per match; off by default).  stage 0: after the filter (alg 1: label
+ * and degree, plus the neighbour-label filter unless fsig_tune.nlf_filter
+ * is 1, over the inverted index, as match/MMatch_kernels.cuh:6-16; alg 2:
+ * the signature filter of match/GSI_kernels.cuh:8-24), stage 1: after
+ * refinement (alg 1; match/MMatch_kernels.cuh:28-117, 190-214) or the same
+ * sets (alg 2).  The set of query vertex u is ascending data vertex ids;
+ * *size is its length and min(size, cap) ids are copied to out (may be
+ * NULL).  FSIG_ENOTFOUND when no set of u is kept. */
+int fsig_set_keep_candidates(fsig_ctx* ctx, int on);
+int fsig_last_candidate_set(const fsig_ctx* ctx, uint32_t u, int stage, uint32_t* out, uint32_t cap,
+                            uint32_t* size);
+
 /* Counts the embeddings of nq queries (alg 1 semantics) against one
  * resident graph.  Filter (with the neighbour-label filter), refinement and
  * candidate edges run for the whole batch at once.  The join is either the
@@ -116,6 +146,22 @@ int fsig_match(fsig_ctx* ctx, const fsig_graph* g, uint32_t k, const int32_t* ql
 int fsig_match_batch(fsig_ctx* ctx, const fsig_graph* g, uint32_t nq, const uint32_t* k,
                      const int32_t* const* qlabels, const uint32_t* const* qoff,
                      const uint32_t* const* qadj, uint64_t* counts_out, double* join_ms);
+/* fsig_match_batch with a bound per query: max_embeddings[i] = 0 counts
+ * query i exactly; otherwise counts_out[i] = min(count, max_embeddings[i])
+ * (fsig_match's max_embeddings: pass L+1 to ask "more than L?") and its
+ * search stops once that many embeddings are found.  A bounded query joins
+ * the lockstep batch until its count reaches the bound or its next level
+ * would exceed 2^20 rows (fsig_tune.spill_log2); it is then finished, from the rows it reached, by
+ * fsig_match's memory-blocked depth-first driver, which stops at the bound.
+ * That driver honours the context's work limit (fsig_set_work_limit): a
+ * query over it is refused, status_out[i] = FSIG_ELIMIT and counts_out[i]
+ * = 0, while the other queries complete (status_out NULL: the call then
+ * returns FSIG_ELIMIT).  No autotuning (lockstep with lanes per query).  No
+ * counterpart in the reference. */
+int fsig_match_batch_bounded(fsig_ctx* ctx, const fsig_graph* g, uint32_t nq, const uint32_t* k,
+                             const int32_t* const* qlabels, const uint32_t* const* qoff,
+                             const uint32_t* const* qadj, const uint64_t* max_embeddings, uint64_t* counts_out,
+                             int32_t* status_out, double* join_ms);
 /* Configuration the autotuner chose for the last batch: engine 1 = lockstep
  * batched join (every query's level p in one count/scan/write sequence),
  * 2 = persistent work-queue walk; lanes per row or group; LDS tile (0:
@@ -173,6 +219,9 @@ typedef struct fsig_tune {
                              least as many neighbours of that label as the
                              query vertex has (graphs with 2..255 labels);
                              1: label and degree only, as the reference */
+  uint32_t spill_log2;    /* fsig_match_batch_bounded: a bounded query leaves
+                             the lockstep join when its next level would
+                             exceed 2^spill_log2 rows (0: 20; 1..40) */
 } fsig_tune;
 int fsig_set_tune(fsig_ctx* ctx, const fsig_tune* t);
 int fsig_get_tune(const fsig_ctx* ctx, fsig_tune* t);
@@ -209,6 +258,7 @@ int fsig_stream_copy(fsig_ctx* ctx, uint64_t bytes, uint32_t iters, double* gbps
  * budget (main/run.cc:101-140, memory_manager/memory_manager.h:220-242). */
 typedef struct fsig_server_stats {
   uint64_t submitted, completed, batches, fallbacks, max_batch_seen;
+  uint64_t refused;        /* requests over the workers' work limit (FSIG_ELIMIT) */
   uint32_t max_concurrent; /* most batches in flight at once */
   uint32_t pad;
   double busy_ms;          /* summed batch wall time over the workers */
@@ -220,6 +270,16 @@ int fsig_server_create(int device, const fsig_graph* g, uint32_t workers, uint64
  * returns its ticket.  The submit time starts the query's latency. */
 int fsig_server_submit(fsig_server* s, uint32_t k, const int32_t* qlabels, const uint32_t* qoff,
                        const uint32_t* qadj, uint64_t* ticket);
+/* fsig_server_submit with a bound: the request's count is min(count,
+ * max_embeddings) (0: exact), as fsig_match_batch_bounded counts it; its
+ * admission projection keeps each stored level under the 2^20 rows after
+ * which a bounded query leaves the lockstep join. */
+int fsig_server_submit_bounded(fsig_server* s, uint32_t k, const int32_t* qlabels, const uint32_t* qoff,
+                               const uint32_t* qadj, uint64_t max_embeddings, uint64_t* ticket);
+/* Work limit (partial embeddings, as fsig_set_work_limit) of every worker;
+ * a request over it completes with FSIG_ELIMIT (refused) and counts in
+ * fsig_server_stats.refused.  Set before submitting. */
+int fsig_server_set_work_limit(fsig_server* s, uint64_t max_rows);
 /* Waits up to timeout_ms (< 0: without limit) for a ticket; returns the
  * query's status (FSIG_EAGAIN: not finished, the ticket stays valid) and on
  * completion its count and submit-to-completion latency.  A completed

@@ -29,6 +29,8 @@ fsig_hgraph* fsig_hg_chung_lu(uint32_t n, uint64_t m, double gamma, uint64_t see
 fsig_hgraph* fsig_hg_rmat(uint32_t scale, uint64_t m, double a, double b, double c, uint64_t seed);
 /* kind 0 uniform, 1 Zipf(s) */
 int fsig_hg_set_labels(fsig_hgraph* g, int kind, int32_t num_labels, double s, uint64_t seed);
+/* 0, or -1 (cannot open), -6 (short write or failed close: the file is
+ * incomplete). */
 int fsig_hg_write(const fsig_hgraph* g, const char* path);
 void fsig_hg_info(const fsig_hgraph* g, uint32_t* n, uint64_t* nnz, int32_t* num_labels, uint32_t* max_degree);
 void fsig_hg_export(const fsig_hgraph* g, uint32_t* off, uint32_t* adj, int32_t* labels);

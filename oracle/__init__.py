@@ -50,6 +50,16 @@ def lib():
                                    _u32p, C.c_uint64, _u64p]
         L.oracle_match_limit.restype = C.c_uint64
         L.oracle_match_limit.argtypes = L.oracle_match.argtypes + [C.c_uint64, C.POINTER(C.c_int)]
+        L.oracle_match_digest.restype = C.c_uint64
+        L.oracle_match_digest.argtypes = [C.c_uint32, _i32p, _u32p, C.c_uint32, _u32p, _u32p, _i32p, C.c_uint32,
+                                          _u64p]
+        L.oracle_row_hash.restype = C.c_uint64
+        L.oracle_row_hash.argtypes = [C.c_uint32, _u32p]
+        L.oracle_triangles.restype = C.c_uint64
+        L.oracle_triangles.argtypes = [C.c_uint32, _u32p, _u32p, C.c_uint32]
+        L.oracle_check_rows.restype = C.c_uint64
+        L.oracle_check_rows.argtypes = [C.c_uint32, _i32p, _u32p, C.c_uint32, _u32p, _u32p, _i32p, _u32p,
+                                        C.c_uint64, C.c_uint32, _u64p]
         _lib = L
     return _lib
 
@@ -140,6 +150,68 @@ def match_partial(qlabels, qadjm, off, adj, labels, threads: int = 1, cap: int =
     cnt = lib().oracle_match_limit(k, _p(ql, _i32p), _p(qa), len(labels), _p(off), _p(adj), _p(labels, _i32p),
                                    threads, _p(rows), cap, C.byref(written), limit, C.byref(stopped))
     return int(cnt), (rows[: written.value] if cap else None), bool(stopped.value)
+
+
+def match_digest(qlabels, qadjm, off, adj, labels, threads: int = 1) -> Tuple[int, Tuple[int, int]]:
+    """(count, digest) of the full embedding set: the digest of
+    fsig_match_digest, computed while enumerating on the host."""
+    off, adj, labels = _csr(off, adj, labels)
+    ql = np.ascontiguousarray(qlabels, dtype=np.int32)
+    qa = np.ascontiguousarray(qadjm, dtype=np.uint32)
+    dig = np.zeros(2, np.uint64)
+    cnt = lib().oracle_match_digest(len(ql), _p(ql, _i32p), _p(qa), len(labels), _p(off), _p(adj),
+                                    _p(labels, _i32p), threads, _p(dig, _u64p))
+    return int(cnt), (int(dig[0]), int(dig[1]))
+
+
+_M64 = (1 << 64) - 1
+
+
+def _mix64_np(z):
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xbf58476d1ce4e5b9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94d049bb133111eb)
+    return z ^ (z >> np.uint64(31))
+
+
+def row_hashes_numpy(rows) -> np.ndarray:
+    """The digest's row hash of each row (query-vertex order), in numpy: an
+    independent restatement of oracle_row_hash."""
+    rows = np.asarray(rows, dtype=np.uint64)
+    n, k = rows.shape
+    with np.errstate(over="ignore"):
+        x = np.full(n, (0x9e3779b97f4a7c15 * (k + 1)) & _M64, dtype=np.uint64)
+        for i in range(k):
+            x = _mix64_np(x + rows[:, i] + np.uint64(i << 32))
+    return x
+
+
+def digest_numpy(rows) -> Tuple[int, int]:
+    """Set digest of explicit rows (numpy restatement)."""
+    x = row_hashes_numpy(rows)
+    with np.errstate(over="ignore"):
+        d1 = _mix64_np(x ^ np.uint64(0xd6e8feb86659fd93))
+    return int(x.sum(dtype=np.uint64)), int(d1.sum(dtype=np.uint64))
+
+
+def triangles(off, adj, threads: int = 1) -> int:
+    """Number of triangles of an undirected CSR graph (each counted once);
+    the unlabeled triangle query has 6 embeddings per triangle."""
+    off = np.ascontiguousarray(off, dtype=np.uint32)
+    adj = np.ascontiguousarray(adj, dtype=np.uint32)
+    return int(lib().oracle_triangles(len(off) - 1, _p(off), _p(adj), threads))
+
+
+def check_rows(qlabels, qadjm, off, adj, labels, rows, threads: int = 1, want_hashes: bool = False):
+    """(number of rows that are not embeddings, row hashes or None)."""
+    off, adj, labels = _csr(off, adj, labels)
+    ql = np.ascontiguousarray(qlabels, dtype=np.int32)
+    qa = np.ascontiguousarray(qadjm, dtype=np.uint32)
+    rows = np.ascontiguousarray(rows, dtype=np.uint32)
+    nrows = rows.shape[0]
+    hashes = np.empty(nrows, np.uint64) if want_hashes else None
+    bad = lib().oracle_check_rows(len(ql), _p(ql, _i32p), _p(qa), len(labels), _p(off), _p(adj), _p(labels, _i32p),
+                                  _p(rows), nrows, threads, _p(hashes, _u64p))
+    return int(bad), hashes
 
 
 # ---------------------------------------------------------------------------

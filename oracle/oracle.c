@@ -27,6 +27,17 @@
  *                       :641-707.  Pinned by closed forms (tr(A^3), 4-cycles,
  *                       K_n counts), brute force over all injective maps on
  *                       small graphs, and networkx VF2 golden fixtures.
+ *   oracle_match_digest the same enumeration folded into the order-independent
+ *                       set digest of fsig_match_digest (row hash: splitmix64
+ *                       finaliser chained over the row in query-vertex
+ *                       order); pinned by a numpy restatement of the digest
+ *                       over the rows oracle_match lists.
+ *   oracle_check_rows   validity of given rows (labels, injectivity, every
+ *                       query edge present) and their row hashes; pinned by
+ *                       planted mistakes in known embeddings.
+ *   oracle_triangles    number of triangles by degree-ordered orientation
+ *                       (each triangle once; the triangle query's embeddings
+ *                       are 6 times it); pinned by oracle_match and tr(A^3).
  * Parity unpinned: none of the functions above.
  */
 #include <pthread.h>
@@ -264,6 +275,7 @@ typedef struct {
   uint64_t limit;
   uint64_t seen;
   int stop;
+  int digest;          /* fold every embedding into dig[] */
 } Match;
 
 static int has_edge(const Match* M, uint32_t a, uint32_t b) {
@@ -281,7 +293,22 @@ typedef struct {
   uint64_t count;
   uint32_t buf[64 * OMAXQ];
   uint32_t nbuf;
+  uint64_t dig[2];
 } Worker;
+
+/* Row hash of the set digest: splitmix64's finaliser (public domain) chained
+ * over the row's words in query-vertex order. */
+static uint64_t omix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+uint64_t oracle_row_hash(uint32_t k, const uint32_t* row) {
+  uint64_t x = 0x9e3779b97f4a7c15ull * (uint64_t)(k + 1);
+  for (uint32_t i = 0; i < k; i++) x = omix64(x + (uint64_t)row[i] + ((uint64_t)i << 32));
+  return x;
+}
 
 static void flush(Worker* W) {
   Match* M = W->M;
@@ -299,6 +326,13 @@ static void emit(Worker* W) {
   if (M->limit && (W->count & 1023u) == 0) {
     uint64_t seen = __atomic_add_fetch(&M->seen, 1024, __ATOMIC_RELAXED);
     if (seen > M->limit) __atomic_store_n(&M->stop, 1, __ATOMIC_RELAXED);
+  }
+  if (M->digest) {
+    uint32_t r[OMAXQ];
+    for (uint32_t p = 0; p < M->k; p++) r[M->order[p]] = W->phi[p];
+    const uint64_t x = oracle_row_hash(M->k, r);
+    W->dig[0] += x;
+    W->dig[1] += omix64(x ^ 0xd6e8feb86659fd93ull);
   }
   if (!M->rows) return;
   uint32_t* r = W->buf + W->nbuf * M->k;
@@ -354,12 +388,15 @@ static void* worker_main(void* arg) {
  * threads > 1 the roots are handed out one at a time to a pthread pool.
  * With limit > 0 the search stops once more than `limit` embeddings were
  * seen; *stopped tells whether it did (the count is then partial). */
-uint64_t oracle_match_limit(uint32_t k, const int32_t* qlabels, const uint32_t* qadjm, uint32_t n,
-                            const uint32_t* off, const uint32_t* adj, const int32_t* labels, uint32_t threads,
-                            uint32_t* rows, uint64_t cap, uint64_t* written, uint64_t limit, int* stopped) {
+static uint64_t match_run(uint32_t k, const int32_t* qlabels, const uint32_t* qadjm, uint32_t n,
+                          const uint32_t* off, const uint32_t* adj, const int32_t* labels, uint32_t threads,
+                          uint32_t* rows, uint64_t cap, uint64_t* written, uint64_t limit, int* stopped,
+                          uint64_t* digest) {
+  if (digest) digest[0] = digest[1] = 0;
   if (k == 0 || k > OMAXQ) return 0;
   Match M;
   memset(&M, 0, sizeof M);
+  M.digest = digest != NULL;
   M.k = k;
   M.n = n;
   M.ql = qlabels;
@@ -404,7 +441,13 @@ uint64_t oracle_match_limit(uint32_t k, const int32_t* qlabels, const uint32_t* 
     for (uint32_t t = 0; t < threads; t++) pthread_join(th[t], NULL);
   }
   uint64_t total = 0;
-  for (uint32_t t = 0; t < threads; t++) total += W[t].count;
+  for (uint32_t t = 0; t < threads; t++) {
+    total += W[t].count;
+    if (digest) {
+      digest[0] += W[t].dig[0];
+      digest[1] += W[t].dig[1];
+    }
+  }
   if (stopped) *stopped = M.stop;
   if (written) *written = M.written;
   pthread_mutex_destroy(&M.lock);
@@ -414,8 +457,155 @@ uint64_t oracle_match_limit(uint32_t k, const int32_t* qlabels, const uint32_t* 
   return total;
 }
 
+uint64_t oracle_match_limit(uint32_t k, const int32_t* qlabels, const uint32_t* qadjm, uint32_t n,
+                            const uint32_t* off, const uint32_t* adj, const int32_t* labels, uint32_t threads,
+                            uint32_t* rows, uint64_t cap, uint64_t* written, uint64_t limit, int* stopped) {
+  return match_run(k, qlabels, qadjm, n, off, adj, labels, threads, rows, cap, written, limit, stopped, NULL);
+}
+
 uint64_t oracle_match(uint32_t k, const int32_t* qlabels, const uint32_t* qadjm, uint32_t n, const uint32_t* off,
                       const uint32_t* adj, const int32_t* labels, uint32_t threads, uint32_t* rows, uint64_t cap,
                       uint64_t* written) {
-  return oracle_match_limit(k, qlabels, qadjm, n, off, adj, labels, threads, rows, cap, written, 0, NULL);
+  return match_run(k, qlabels, qadjm, n, off, adj, labels, threads, rows, cap, written, 0, NULL, NULL);
+}
+
+/* Count and set digest (digest[0] = sum of row hashes x, digest[1] = sum of
+ * omix64(x ^ salt), both mod 2^64) of every embedding. */
+uint64_t oracle_match_digest(uint32_t k, const int32_t* qlabels, const uint32_t* qadjm, uint32_t n,
+                             const uint32_t* off, const uint32_t* adj, const int32_t* labels, uint32_t threads,
+                             uint64_t* digest) {
+  return match_run(k, qlabels, qadjm, n, off, adj, labels, threads, NULL, 0, NULL, 0, NULL, digest);
+}
+
+/* ---------------------------------------------------------------------------
+ * Checks rows (k words each, query-vertex order) against the contract:
+ * every word a vertex of the graph with the query vertex's label, the words
+ * distinct, and every query edge a data edge.  Returns the number of rows
+ * that fail; hashes (may be NULL) receives each row's oracle_row_hash. */
+typedef struct {
+  uint32_t k, n;
+  const int32_t* ql;
+  const uint32_t* qadjm;
+  const uint32_t* off;
+  const uint32_t* adj;
+  const int32_t* labels;
+  const uint32_t* rows;
+  uint64_t nrows;
+  uint64_t* hashes;
+  uint64_t next;
+  uint64_t bad;
+} Check;
+
+static int row_ok(const Check* K, const uint32_t* r) {
+  Match M;
+  M.off = K->off;
+  M.adj = K->adj;
+  for (uint32_t i = 0; i < K->k; i++) {
+    if (r[i] >= K->n || K->labels[r[i]] != K->ql[i]) return 0;
+    for (uint32_t j = 0; j < i; j++)
+      if (r[j] == r[i]) return 0;
+  }
+  for (uint32_t i = 0; i < K->k; i++)
+    for (uint32_t j = i + 1; j < K->k; j++)
+      if (((K->qadjm[i] >> j) & 1u) && !has_edge(&M, r[i], r[j])) return 0;
+  return 1;
+}
+
+static void* check_main(void* arg) {
+  Check* K = (Check*)arg;
+  uint64_t bad = 0;
+  for (;;) {
+    uint64_t at = __atomic_fetch_add(&K->next, 65536, __ATOMIC_RELAXED);
+    if (at >= K->nrows) break;
+    uint64_t end = at + 65536 < K->nrows ? at + 65536 : K->nrows;
+    for (uint64_t r = at; r < end; r++) {
+      const uint32_t* row = K->rows + r * K->k;
+      bad += !row_ok(K, row);
+      if (K->hashes) K->hashes[r] = oracle_row_hash(K->k, row);
+    }
+  }
+  __atomic_fetch_add(&K->bad, bad, __ATOMIC_RELAXED);
+  return NULL;
+}
+
+uint64_t oracle_check_rows(uint32_t k, const int32_t* qlabels, const uint32_t* qadjm, uint32_t n,
+                           const uint32_t* off, const uint32_t* adj, const int32_t* labels, const uint32_t* rows,
+                           uint64_t nrows, uint32_t threads, uint64_t* hashes) {
+  Check K;
+  memset(&K, 0, sizeof K);
+  K.k = k; K.n = n; K.ql = qlabels; K.qadjm = qadjm; K.off = off; K.adj = adj; K.labels = labels;
+  K.rows = rows; K.nrows = nrows; K.hashes = hashes;
+  if (threads < 1) threads = 1;
+  pthread_t* th = (pthread_t*)calloc(threads, sizeof(pthread_t));
+  for (uint32_t t = 0; t < threads; t++) pthread_create(&th[t], NULL, check_main, &K);
+  for (uint32_t t = 0; t < threads; t++) pthread_join(th[t], NULL);
+  free(th);
+  return K.bad;
+}
+
+/* ---------------------------------------------------------------------------
+ * Triangles by degree-ordered orientation: each edge points from the vertex
+ * of lower (degree, id) rank to the higher; a triangle is counted once, at
+ * its lowest vertex v and middle vertex u, as a common out-neighbour of both. */
+typedef struct {
+  uint32_t n;
+  const uint64_t* ooff;
+  const uint32_t* oadj;
+  uint32_t next;
+  uint64_t total;
+} Tri;
+
+static int rank_less(const uint32_t* off, uint32_t a, uint32_t b) {
+  uint32_t da = off[a + 1] - off[a], db = off[b + 1] - off[b];
+  return da < db || (da == db && a < b);
+}
+
+static void* tri_main(void* arg) {
+  Tri* T = (Tri*)arg;
+  uint64_t cnt = 0;
+  for (;;) {
+    uint32_t v0 = __atomic_fetch_add(&T->next, 256, __ATOMIC_RELAXED);
+    if (v0 >= T->n) break;
+    uint32_t v1 = v0 + 256 < T->n ? v0 + 256 : T->n;
+    for (uint32_t v = v0; v < v1; v++)
+      for (uint64_t i = T->ooff[v]; i < T->ooff[v + 1]; i++) {
+        uint32_t u = T->oadj[i];
+        uint64_t a = T->ooff[v], ae = T->ooff[v + 1], b = T->ooff[u], be = T->ooff[u + 1];
+        while (a < ae && b < be) {
+          if (T->oadj[a] < T->oadj[b]) a++;
+          else if (T->oadj[a] > T->oadj[b]) b++;
+          else { cnt++; a++; b++; }
+        }
+      }
+  }
+  __atomic_fetch_add(&T->total, cnt, __ATOMIC_RELAXED);
+  return NULL;
+}
+
+uint64_t oracle_triangles(uint32_t n, const uint32_t* off, const uint32_t* adj, uint32_t threads) {
+  uint64_t* ooff = (uint64_t*)calloc((size_t)n + 1, sizeof(uint64_t));
+  for (uint32_t v = 0; v < n; v++) {
+    uint64_t d = 0;
+    for (uint32_t i = off[v]; i < off[v + 1]; i++) d += rank_less(off, v, adj[i]);
+    ooff[v + 1] = ooff[v] + d;
+  }
+  uint32_t* oadj = (uint32_t*)malloc(sizeof(uint32_t) * (ooff[n] ? ooff[n] : 1));
+  for (uint32_t v = 0; v < n; v++) {
+    uint64_t at = ooff[v];
+    for (uint32_t i = off[v]; i < off[v + 1]; i++)
+      if (rank_less(off, v, adj[i])) oadj[at++] = adj[i]; /* ascending ids, as adj */
+  }
+  Tri T;
+  memset(&T, 0, sizeof T);
+  T.n = n;
+  T.ooff = ooff;
+  T.oadj = oadj;
+  if (threads < 1) threads = 1;
+  pthread_t* th = (pthread_t*)calloc(threads, sizeof(pthread_t));
+  for (uint32_t t = 0; t < threads; t++) pthread_create(&th[t], NULL, tri_main, &T);
+  for (uint32_t t = 0; t < threads; t++) pthread_join(th[t], NULL);
+  free(th);
+  free(ooff);
+  free(oadj);
+  return T.total;
 }

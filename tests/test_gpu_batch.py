@@ -16,6 +16,7 @@ def _mixed_queries(g, n, seed):
     qs.append(F.Query(1, [2], np.zeros((0, 2))))                      # single vertex
     qs.append(F.Query(4, [0, 1, 0, 1], [(0, 1), (2, 3)]))             # disconnected
     qs.append(F.Query(3, [0, 99, 1], [(0, 1), (1, 2)]))               # label absent from the graph
+    qs.append(F.Query(2, [0x7fffffff, 1], [(0, 1)]))                  # label far past the inverted index
     k = 9
     qs.append(F.Query(k, [0] * k, [(a, b) for a in range(k) for b in range(a + 1, k)]))  # no host
     qs.append(F.Query(2, [1, 1], [(0, 1)]))
@@ -116,3 +117,54 @@ def test_batch_hub_candidates(gpu_ctx):
             gpu_ctx.set_tune()
         assert counts.tolist() == list(ref), strategy
     dg.free()
+
+
+@pytest.mark.parametrize("spill_log2", [0, 4])
+def test_bounded_batch_equals_bounded_plain_and_oracle(gpu_ctx, spill_log2):
+    """fsig_match_batch_bounded: min(count, bound) per query, equal to
+    fsig_match's bounded count and to the oracle (stopped searches prove
+    the count exceeds the bound).  spill_log2 = 4 sends every bounded query
+    with a level above 16 rows to the depth-first driver; exact queries
+    (bound 0) stay in the lockstep join."""
+    g = F.HostGraph.chung_lu(20000, 120000, 2.3, 97).set_labels("zipf", 6, 1.2, 98)
+    off, adj, lab = g.arrays()
+    dg = gpu_ctx.upload(g)
+    qs = [q for _, q in W.walk_queries(g, 40, 3, 12, 21)] + _mixed_queries(g, 0, 3)
+    rng = np.random.default_rng(5)
+    bounds = [int(b) for b in rng.choice([0, 1, 1000, 10 ** 5, 10 ** 6], size=len(qs))]
+    for i, q in enumerate(qs):  # exact queries must stay small enough for the oracle
+        if bounds[i] == 0 and F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT, max_embeddings=10 ** 7 + 1).count > 10 ** 7:
+            bounds[i] = 10 ** 6
+    try:
+        gpu_ctx.set_tune(spill_log2=spill_log2)
+        counts, _, st = F.match_batch(dg, qs, bounds=bounds, status=True)
+    finally:
+        gpu_ctx.set_tune()
+    assert not st.any()
+    for q, b, c in zip(qs, bounds, counts.tolist()):
+        plain = F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT, max_embeddings=b).count
+        assert c == plain, (q.k, q.edges.tolist(), b)
+        want, _, stopped = O.match_partial(q.labels, q.adjm(), off, adj, lab, threads=8, limit=b)
+        if b == 0 or not stopped:
+            assert c == (min(want, b) if b else want), (q.k, b)
+        else:
+            assert c == b  # the oracle saw more than b embeddings
+
+
+def test_bounded_batch_work_limit_refuses_per_query(gpu_ctx):
+    """A bounded query over the context's work limit is refused alone
+    (FSIG_ELIMIT in its status); the others complete."""
+    g = F.HostGraph.chung_lu(20000, 160000, 2.2, 99).set_labels("uniform", 2, seed=100)
+    dg = gpu_ctx.upload(g)
+    heavy = F.Query(6, [0, 1, 0, 1, 0, 1], [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5)])
+    light = F.Query(2, [0, 1], [(0, 1)])
+    want_light = F.match(dg, light, F.ALG_MMATCH, F.MODE_COUNT).count
+    try:
+        gpu_ctx.set_tune(spill_log2=4)
+        gpu_ctx.set_work_limit(1000)
+        counts, _, st = F.match_batch(dg, [heavy, light], bounds=[10 ** 12, 10 ** 12], status=True)
+    finally:
+        gpu_ctx.set_work_limit(0)
+        gpu_ctx.set_tune()
+    assert st.tolist() == [F.FSIG_ELIMIT, 0]
+    assert counts.tolist() == [0, want_light]

@@ -93,21 +93,59 @@ def test_unlabeled_patterns_count(gpu_ctx):
         dg.free()
 
 
-def test_candidate_sets_equal_oracle_refinement(gpu_ctx):
+def _deg(q):
+    return [int(bin(int(m)).count("1")) for m in q.adjm()]
+
+
+def test_candidate_sets_equal_oracle_filter_and_refinement(gpu_ctx):
+    """Element by element: the label/degree filter's sets equal oracle_ldf and
+    the refined sets equal oracle_refine (the reference's filters only)."""
     g = F.HostGraph.chung_lu(8000, 40000, 2.5, 31).set_labels("uniform", 4, seed=32)
     off, adj, lab = g.arrays()
     dg = gpu_ctx.upload(g)
     passes = gpu_ctx.get_tune()["refine_passes"] or 4
-    gpu_ctx.set_tune(nlf_filter=1)  # the reference's filters only
+    gpu_ctx.set_tune(nlf_filter=1)
+    gpu_ctx.keep_candidates(True)
     try:
-        for s in range(5):
+        shrunk = False
+        for s in range(6):
             q = g.walk_query(6, 5, 12, seed=500 + s)
             F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT)
-            got = gpu_ctx.last_candidates(q.k)
-            want = O.refine(q.labels, q.adjm(), off, adj, lab, passes)
-            assert np.array_equal(got, want)
+            sizes, sets = O.refine(q.labels, q.adjm(), off, adj, lab, passes, want_sets=True)
+            deg = _deg(q)
+            for u in range(q.k):
+                ldf = gpu_ctx.last_candidate_set(u, 0)
+                assert np.array_equal(ldf, O.ldf(off, lab, int(q.labels[u]), deg[u])), (s, u)
+                got = gpu_ctx.last_candidate_set(u, 1)
+                assert np.array_equal(got, np.nonzero(sets[u])[0]), (s, u)
+                shrunk |= len(got) < len(ldf)
+            assert np.array_equal(gpu_ctx.last_candidates(q.k), sizes)
+        assert shrunk  # refinement removed candidates somewhere
     finally:
+        gpu_ctx.keep_candidates(False)
         gpu_ctx.set_tune()
+
+
+def test_candidate_sets_neighbour_label_filter_bounds(gpu_ctx):
+    """With the neighbour-label filter on, each filtered set lies inside the
+    label/degree set and holds every embedding's image."""
+    g = F.HostGraph.chung_lu(8000, 40000, 2.5, 35).set_labels("uniform", 4, seed=36)
+    off, adj, lab = g.arrays()
+    dg = gpu_ctx.upload(g)
+    gpu_ctx.keep_candidates(True)
+    try:
+        for s in range(4):
+            q = g.walk_query(5, 4, 9, seed=600 + s)
+            cnt, rows = O.match(q.labels, q.adjm(), off, adj, lab, threads=8, cap=10 ** 7)
+            assert F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT).count == cnt
+            deg = _deg(q)
+            for u in range(q.k):
+                for stage in (0, 1):
+                    got = gpu_ctx.last_candidate_set(u, stage)
+                    assert np.isin(got, O.ldf(off, lab, int(q.labels[u]), deg[u])).all()
+                    assert np.isin(rows[:, u], got).all()
+    finally:
+        gpu_ctx.keep_candidates(False)
 
 
 def test_neighbour_label_filter_is_sound_and_prunes(gpu_ctx):
@@ -137,16 +175,70 @@ def test_neighbour_label_filter_is_sound_and_prunes(gpu_ctx):
 
 
 def test_gsi_candidate_sets_equal_oracle_signature_filter(gpu_ctx):
+    """Element by element: alg 2's candidate sets equal oracle_sig_filter."""
     g = F.HostGraph.chung_lu(8000, 40000, 2.5, 41).set_labels("zipf", 5, 1.2, 42)
     dg = gpu_ctx.upload(g, pcsr=True)
     dsig = g.signature(True)
-    for s in range(4):
-        q = g.walk_query(5, 4, 10, seed=700 + s)
-        F.match(dg, q, F.ALG_GSI, F.MODE_COUNT)
-        got = gpu_ctx.last_candidates(q.k)
-        qsig = q.signature().reshape(q.k, 16)
-        want = [len(O.sig_filter(dsig, qsig[u])) for u in range(q.k)]
-        assert got.tolist() == want
+    gpu_ctx.keep_candidates(True)
+    try:
+        for s in range(4):
+            q = g.walk_query(5, 4, 10, seed=700 + s)
+            F.match(dg, q, F.ALG_GSI, F.MODE_COUNT)
+            qsig = q.signature().reshape(q.k, 16)
+            for u in range(q.k):
+                assert np.array_equal(gpu_ctx.last_candidate_set(u, 1), O.sig_filter(dsig, qsig[u])), (s, u)
+    finally:
+        gpu_ctx.keep_candidates(False)
+
+
+def test_reference_unit_ids_with_colliding_labels(gpu_ctx):
+    """PCSR units numbered as the reference numbers them (graph/graph.h:
+    107-122), where (1,0) and (0,0), (2,0) and (1,1) share a unit: alg 2's
+    sets still equal the oracle's."""
+    g = F.HostGraph.chung_lu(6000, 30000, 2.5, 43).set_labels("uniform", 3, seed=44)
+    off, adj, lab = g.arrays()
+    dg = gpu_ctx.upload(g, pcsr=True, reference_unit_ids=True)
+    pairs = set()
+    queries = [F.Query(3, [1, 0, 0], [(0, 1), (1, 2)]), F.Query(3, [2, 0, 1], [(0, 1), (1, 2), (0, 2)]),
+               F.Query(4, [1, 1, 0, 2], [(0, 1), (1, 2), (2, 3), (0, 3)])]
+    queries += [g.walk_query(5, 4, 8, seed=800 + s) for s in range(4)]
+    for q in queries:
+        for a, b in q.edges.tolist():
+            pairs.add(tuple(sorted((int(q.labels[a]), int(q.labels[b])))))
+        want = O.match_digest(q.labels, q.adjm(), off, adj, lab, threads=8)
+        if want[0] <= 2_000_000:
+            _check_set(dg, q, F.ALG_GSI, off, adj, lab)
+        assert F.match_digest(dg, q, F.ALG_GSI) == want
+        assert F.match(dg, q, F.ALG_GSI, F.MODE_COUNT).count == want[0]
+    assert (0, 1) in pairs and (0, 0) in pairs and (0, 2) in pairs and (1, 1) in pairs
+
+
+def test_digest_equals_oracle_and_enumerated_rows(gpu_ctx):
+    """fsig_match_digest equals the oracle's digest of the same set and the
+    digest of the rows fsig_match enumerates, for both algorithms."""
+    g = F.HostGraph.chung_lu(20000, 120000, 2.3, 45).set_labels("zipf", 6, 1.2, 46)
+    off, adj, lab = g.arrays()
+    dg = gpu_ctx.upload(g, pcsr=True)
+    done = 0
+    for s, (k, lo, hi) in enumerate([(4, 3, 6), (5, 4, 8), (6, 7, 12), (7, 6, 9)] * 6):
+        if done == 4:
+            break
+        try:
+            q = g.walk_query(k, lo, hi, seed=900 + s, max_attempts=20000)
+        except F.FsigError:
+            continue
+        if O.match(q.labels, q.adjm(), off, adj, lab, threads=8, limit=50_000_000)[0] == O.TOO_MANY:
+            continue
+        print(f"digest: k={k} seed={900 + s}", flush=True)
+        want = O.match_digest(q.labels, q.adjm(), off, adj, lab, threads=8)
+        assert want[0] > 0
+        done += 1
+        for alg in (F.ALG_MMATCH, F.ALG_GSI):
+            assert F.match_digest(dg, q, alg) == want, (s, alg)
+        if want[0] <= 2_000_000:
+            r = F.match(dg, q, F.ALG_MMATCH, F.MODE_ENUMERATE, emb_cap=want[0])
+            assert O.digest_numpy(r.embeddings) == want[1]
+    assert done == 4
 
 
 def test_edge_cases(gpu_ctx):
@@ -184,14 +276,15 @@ def test_edge_cases(gpu_ctx):
 
 def test_bounded_count(gpu_ctx):
     g = F.HostGraph.chung_lu(5000, 30000, 2.3, 63).set_labels("uniform", 2, seed=64)
-    dg = gpu_ctx.upload(g)
+    off, adj, lab = g.arrays()
+    dg = gpu_ctx.upload(g, pcsr=True)
     q = F.Query(4, [0, 1, 0, 1], [(0, 1), (1, 2), (2, 3)])
-    total = F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT).count
+    total, _ = O.match(q.labels, q.adjm(), off, adj, lab, threads=8)
     assert total > 1000
-    for bound in (1, 999, total - 1, total, total + 1):
-        for alg in (F.ALG_MMATCH, F.ALG_GSI):
-            pass
-        assert F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT, max_embeddings=bound).count == min(total, bound)
+    for alg in (F.ALG_MMATCH, F.ALG_GSI):
+        assert F.match(dg, q, alg, F.MODE_COUNT).count == total
+        for bound in (1, 999, total - 1, total, total + 1):
+            assert F.match(dg, q, alg, F.MODE_COUNT, max_embeddings=bound).count == min(total, bound), (alg, bound)
 
 
 def test_bounded_enumeration(gpu_ctx):

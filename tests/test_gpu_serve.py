@@ -24,6 +24,7 @@ def _queries(g, dg, n, seed):
     qs = [q for _, q in W.walk_queries(g, n, 3, 12, seed, dg=dg, max_count=2_000_000, max_rows=4_000_000)]
     qs.append(F.Query(1, [2], np.zeros((0, 2))))
     qs.append(F.Query(3, [0, 99, 1], [(0, 1), (1, 2)]))  # label absent: count 0
+    qs.append(F.Query(2, [0x7fffffff, 1], [(0, 1)]))  # label far past the inverted index: count 0
     qs.append(F.Query(4, [0, 1, 0, 1], [(0, 1), (2, 3)]))  # disconnected
     return qs
 
@@ -118,3 +119,19 @@ def test_server_refuses_more_than_4096_labels(gpu_ctx):
     q = g.walk_query(3, 2, 3, seed=4)
     assert F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT).count >= 1
     dg.free()
+
+
+def test_served_bounded_equals_alone(gpu_ctx):
+    """Bounded requests: the served count is fsig_match's min(count, bound)."""
+    g = _graph(3)
+    dg = gpu_ctx.upload(g)
+    qs = [q for _, q in W.walk_queries(g, 30, 3, 12, 17)]
+    bounds = [[0, 10, 10 ** 4, 10 ** 6][i % 4] for i in range(len(qs))]
+    for i, q in enumerate(qs):
+        if bounds[i] == 0 and F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT, max_embeddings=10 ** 8 + 1).count > 10 ** 8:
+            bounds[i] = 10 ** 6
+    alone = [F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT, max_embeddings=b).count for q, b in zip(qs, bounds)]
+    with F.Server(dg, workers=3, hbm_budget=1 << 30, max_batch=8) as srv:
+        tickets = [srv.submit(q, b) for q, b in zip(qs, bounds)]
+        got = [srv.wait(t)[0] for t in tickets]
+    assert got == alone

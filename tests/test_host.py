@@ -302,3 +302,17 @@ def test_embedding_file_errors(tmp_path):
     assert F.host_lib().fsig_write_embeddings(str(tmp_path).encode(), 3, 4, ptr, 4) == -1
     if os.path.exists("/dev/full"):  # every write to it fails
         assert F.host_lib().fsig_write_embeddings(b"/dev/full", 3, 4, ptr, 4) == -6
+
+
+def test_graph_file_write_errors(tmp_path):
+    g = F.HostGraph.chung_lu(2000, 8000, 2.5, 5).set_labels("uniform", 3, seed=6)
+    with pytest.raises(F.FsigError) as e:
+        g.write(str(tmp_path))  # a directory cannot be opened for writing
+    assert e.value.code == -1
+    if os.path.exists("/dev/full"):  # every write to it fails: the file would be truncated
+        with pytest.raises(F.FsigError) as e:
+            g.write("/dev/full")
+        assert e.value.code == -6
+    path = str(tmp_path / "ok.graph")
+    g.write(path)
+    assert F.HostGraph.read(path).n == g.n
