@@ -118,7 +118,7 @@ HIP_SYMBOLS = [
     "fsig_server_get_stats", "fsig_server_projected", "fsig_server_last_error", "fsig_server_destroy",
     "fsig_project_bytes", "fsig_graph_label_pairs", "fsig_match_digest", "fsig_set_keep_candidates",
     "fsig_last_candidate_set", "fsig_match_batch_bounded", "fsig_server_submit_bounded",
-    "fsig_server_set_work_limit",
+    "fsig_server_set_work_limit", "fsig_match_noalloc",
 ]
 
 
@@ -166,6 +166,7 @@ def hip_lib():
         L.fsig_set_work_limit.argtypes = [_vp, C.c_uint64]
         L.fsig_match_digest.argtypes = [_vp, _vp, C.c_uint32, _i32p, _u32p, _u32p, _u32p, C.c_int, _u64p, _u64p]
         L.fsig_set_keep_candidates.argtypes = [_vp, C.c_int]
+        L.fsig_match_noalloc.argtypes = L.fsig_match.argtypes
         L.fsig_last_candidate_set.argtypes = [_vp, C.c_uint32, C.c_int, _u32p, C.c_uint32, _u32p]
         _optional_bindings(L)
         _hip = L

@@ -2,6 +2,7 @@
 // entry points of include/fsig.h.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "device.h"
 #include "internal.h"
@@ -423,6 +424,83 @@ int fsig_match_digest(fsig_ctx* c, const fsig_graph* g, uint32_t k, const int32_
   if (c && !digest_out) return set_err(c, FSIG_EINVAL, "digest: null digest");
   return match_impl(c, g, k, qlabels, qoff, qadj, qsig, alg, FSIG_MODE_ENUMERATE, 0, count_out, nullptr, 0,
                     digest_out);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Static scratch of fsig_match_noalloc: the device counters (first 512
+// bytes) and the arena of one match.  Each device that loads the library
+// holds its own copy until the process ends.  In host code `scratch` names a
+// host-side stand-in, never the device memory: its device address is taken
+// once per device with hipGetSymbolAddress and only that address is used.
+constexpr size_t kScratchCounters = 64 * sizeof(unsigned long long);
+static __device__ __align__(256) char scratch[FSIG_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// Device address of `scratch` on the context's device (looked up once per
+// device, with that device current).
+int scratch_of(fsig_ctx* c, char** out) {
+  if (c->device < 0 || c->device >= kMaxDevices) return set_err(c, FSIG_EUNSUPPORTED, "noalloc: device id too large");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[c->device]) {
+    FSIG_HIP(c, hipSetDevice(c->device));
+    void* p = nullptr;
+    FSIG_HIP(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_err(c, FSIG_EHIP, "noalloc: no device address for the scratch array");
+    g_scratch_ptr[c->device] = (char*)p;
+  }
+  *out = g_scratch_ptr[c->device];
+  return FSIG_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int fsig_match_noalloc(fsig_ctx* c, const fsig_graph* g, uint32_t k, const int32_t* qlabels, const uint32_t* qoff,
+                       const uint32_t* qadj, const uint32_t* qsig, int alg, int mode, uint64_t max_emb,
+                       uint64_t* count_out, uint32_t* emb_out, uint64_t emb_cap) {
+  if (!c) return FSIG_EINVAL;
+  if (!g || !count_out) return set_err(c, FSIG_EINVAL, "noalloc: null argument");
+  // the candidate mask (4 bytes a vertex) and the candidate arrays (at most
+  // 4 bytes a vertex of the query's labels, twice) must fit with room left
+  const uint64_t need = 12ull * g->n + (1u << 20);
+  if (need > FSIG_NOALLOC_SCRATCH_BYTES - kScratchCounters)
+    return set_err(c, FSIG_EINVAL,
+                   "noalloc: a graph of " + std::to_string(g->n) + " vertices needs at least " +
+                       std::to_string(need) + " bytes of scratch, more than the static " +
+                       std::to_string((unsigned long long)FSIG_NOALLOC_SCRATCH_BYTES) + " bytes");
+  char* base = nullptr;
+  int rc = scratch_of(c, &base);
+  if (rc) return rc;
+  // swap the context's counters and arena for the scratch for this call; no
+  // event is recorded (profiling off), nothing is allocated on the device
+  Arena saved_arena = c->arena;
+  unsigned long long* saved_counters = c->d_counters;
+  const bool saved_prof = c->profiling;
+  c->d_counters = (unsigned long long*)base;
+  c->arena = Arena();
+  c->arena.base = base + kScratchCounters;
+  c->arena.cap = FSIG_NOALLOC_SCRATCH_BYTES - kScratchCounters;
+  c->profiling = false;
+  rc = FSIG_OK;
+  // the counters must start at zero: earlier calls leave theirs behind
+  if (hipSetDevice(c->device) != hipSuccess) rc = set_err(c, FSIG_EHIP, "noalloc: hipSetDevice failed");
+  if (rc == FSIG_OK) {
+    k_fill_words<<<1, 64, 0, c->stream>>>((uint32_t*)base, (uint32_t)(kScratchCounters / 4));
+    if (hipGetLastError() != hipSuccess) rc = set_err(c, FSIG_EHIP, "noalloc: clearing the scratch counters failed");
+  }
+  if (rc == FSIG_OK)
+    rc = match_impl(c, g, k, qlabels, qoff, qadj, qsig, alg, mode, max_emb, count_out, emb_out, emb_cap, nullptr);
+  if (rc == FSIG_ENOMEM) c->err = "noalloc: the static scratch is exhausted (" + c->err + ")";
+  c->arena = saved_arena;
+  c->d_counters = saved_counters;
+  c->profiling = saved_prof;
+  return rc;
 }
 
 int fsig_set_keep_candidates(fsig_ctx* c, int on) {

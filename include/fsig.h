@@ -117,6 +117,27 @@ int fsig_match_digest(fsig_ctx* ctx, const fsig_graph* g, uint32_t k, const int3
                       const uint32_t* qoff, const uint32_t* qadj, const uint32_t* qsig, int alg,
                       uint64_t* count_out, uint64_t* digest_out);
 
+/* fsig_match for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  The match synchronises its stream between levels, so the call
+ * cannot itself be captured into a HIP graph.  Same arguments, results
+ * and errors as fsig_match, but every buffer of the match beyond its inputs
+ * and outputs (candidate masks and arrays, candidate-edge lists, join
+ * tables, scan partials, counters, the staging of emb_out) comes from one
+ * static device array of FSIG_NOALLOC_SCRATCH_BYTES, which each GPU that
+ * loads the library holds until the process ends; the call allocates no
+ * device memory and records no profiling events.  The graph and the
+ * context's stream and pinned host staging are the caller's, made before.
+ * A graph too large for the scratch is rejected with FSIG_EINVAL (at least
+ * 12 bytes a vertex plus 1 MiB must fit); a match that runs out of scratch
+ * returns FSIG_ENOMEM.  All calls on one GPU share its scratch, which keeps
+ * the last call's contents: calls that overlap on one GPU (any contexts or
+ * threads) race and must be serialised by the caller. */
+#define FSIG_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int fsig_match_noalloc(fsig_ctx* ctx, const fsig_graph* g, uint32_t k, const int32_t* qlabels,
+                       const uint32_t* qoff, const uint32_t* qadj, const uint32_t* qsig, int alg, int mode,
+                       uint64_t max_embeddings, uint64_t* count_out, uint32_t* emb_out, uint64_t emb_cap);
+
 /* Candidate sets of the last fsig_match / fsig_match_digest of a context,
  * kept on the host while fsig_set_keep_candidates(ctx, 1) is in force (a
  * copy per match; off by default).  stage 0: after the filter (alg 1: label
