@@ -1,26 +1,46 @@
 """Benchmark of the MI355X subgraph-matching engine.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
-    python bench.py --workload c3|c4|c5      (other GPU workloads)
-    python bench.py --workload ingest        (host ingest, no GPU)
+    python bench.py [--gpus 1] [--steps K] [--warmup W]       (all three GPU workloads, one JSON line)
+    python bench.py --workload c2|c4|c5|c3                    (one GPU workload)
+    python bench.py --workload ingest                         (host ingest, no GPU)
 
-A step is one pass of the hot path over the C2 workload (the dblp-shaped
-configuration the headline metric is quoted on): a Chung-Lu data graph of
-317,080 vertices / 1,049,866 edges with 16 Zipf(1.2) labels, resident in HBM,
-and its two 8-vertex random-walk queries (dense and sparse), each matched by
-alg 1 (fsig_match, FSIG_MODE_COUNT: every embedding is enumerated level by
-level; the last level is counted, not written).  Inputs are resident before
-the timed region; every fsig_match returns only after its stream drained.
+The default run prints ONE JSON line whose headline is the served workload
+(C5), the largest single-GPU configuration of the metric "embeddings/s &
+queries/s (batched vs per-query), p50/p99 latency, % HBM peak":
 
-Prints one JSON line.  `value` is embeddings/s of the timed steps; `roofline`
-describes the dominant kernel family measured with HIP events on the
-engine's stream during the timed region; `cpu_baseline` is the oracle
-(oracle/oracle.c, a restatement, "port") on the host cores with a bounded
-sample of the same queries.
+  C5  served: an R-MAT graph of 2^24 vertices / 200M edges with 64 uniform
+      labels, resident in HBM; a seeded Poisson stream of 128 random-walk
+      count queries (k = round(exp(U(ln 4, ln 32)))), every drawn query kept,
+      each counted to a bound of 1e8 embeddings (min(count, 1e8), exact
+      below it), a query over the work limit (2e9 partial embeddings)
+      refused and reported.  A step serves the whole stream through the
+      continuous-batching server (fsig_server_*); `value` is requests/s over
+      the K timed steps, with p50/p99 arrival-to-completion latency.  The
+      same stream through one-at-a-time fsig_match calls in arrival order is
+      the plain baseline (served_over_plain).
+  C4  batched: a Chung-Lu graph of 1M vertices / 10M edges with 32 Zipf(1.2)
+      labels; the first 256 random-walk queries (k uniform in 4..16, dense
+      and sparse alternating), bounded and refused as C5's; one
+      fsig_match_batch_bounded call per step against the loop of fsig_match
+      calls (batched_over_plain), counts equal.
+  C2  per query: the dblp-shaped Chung-Lu graph (317,080 V / 1,049,866 E, 16
+      Zipf(1.2) labels) and its 8-vertex dense and sparse random-walk
+      queries counted exactly by alg 1; embeddings/s, and the roofline of
+      the dominant kernel family (HIP events on the engine's stream over the
+      timed steps; algorithmic bytes counted on the device) with the HBM
+      traffic rocprofv3 measured for it (profiles/).  `cpu_baseline` is the
+      oracle (oracle/oracle.c, a restatement: "port") on the host's cores
+      over a bounded sample of the sparse query: one warm-up, the median of
+      5 runs, on all cores and on one.
+
+Inputs are resident in HBM before every timed region; every call returns
+after its stream drained; results are compared across engines before they
+are reported.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -32,7 +52,7 @@ HBM_PEAK_GBS = 8000.0
 KERNEL_OF_FAMILY = {"join_tail": "k_join_tail (fused last two levels)", "join_count": "k_join (count pass)",
                     "join_write": "k_join (write pass)", "refine": "k_refine", "cedge": "k_cedge_count/fill",
                     "filter": "k_ldf", "batch": "k_join_multi"}
-COUNTERS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "c2_hbm_counters.json")
+COUNTERS = os.path.join(ROOT, "profiles", "c2_hbm_counters.json")
 
 
 def measured_traffic(kernel):
@@ -49,11 +69,19 @@ def measured_traffic(kernel):
 
 
 def log(*a):
-    print(*a, file=sys.stderr, flush=True)
+    print(time.strftime("%H:%M:%S"), *a, file=sys.stderr, flush=True)
 
 
 def cpu_model():
-    """The host CPU's model name, from /proc/cpuinfo (None where unreadable)."""
+    """The host CPU's model name (lscpu, else /proc/cpuinfo; None where
+    neither answers)."""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
@@ -64,105 +92,68 @@ def cpu_model():
     return None
 
 
-def cpu_baseline(g, queries, seconds_target=20.0):
-    """Oracle on the host: the dense query is counted in full (it also checks
-    the GPU count at full size); the sparse query is sampled: the search
-    stops after a bounded number of embeddings, sized from a short probe to
-    take about the rest of the time budget."""
+def pct(a, q):
+    import numpy as np
+    return round(float(np.percentile(np.asarray(a, dtype=float), q)), 3) if len(a) else None
+
+
+# ---------------------------------------------------------------------------
+# C2: per-query embeddings/s, roofline, CPU baseline
+def cpu_baseline(g, queries, seconds_target=4.0):
+    """The oracle on the host over a bounded sample of the sparse query (the
+    search stops after `limit` embeddings, sized by a probe to take about
+    seconds_target on all cores): one warm-up, then the median of 5 runs, on
+    all cores and on one thread (a limit 1/16 as large).  The dense query is
+    also counted in full once, as a check of the device count."""
+    import numpy as np
     import oracle as O
     off, adj, lab = g.arrays()
     threads = max(1, min(16, os.cpu_count() or 1))
-    total_emb, total_s, parts, checks = 0, 0.0, [], {}
-    for name, q in queries:
-        if name == "dense":
-            t = time.perf_counter()
-            cnt, _, stopped = O.match_partial(q.labels, q.adjm(), off, adj, lab, threads=threads, limit=4_000_000_000)
-            dt = time.perf_counter() - t
-            total_emb += cnt
-            total_s += dt
-            parts.append(f"{name}: {'first ' if stopped else 'all '}{cnt} embeddings")
-            if not stopped:
-                checks[name] = cnt
-        else:
-            t = time.perf_counter()
-            probe, _, _ = O.match_partial(q.labels, q.adjm(), off, adj, lab, threads=threads, limit=20_000_000)
-            rate = probe / max(time.perf_counter() - t, 1e-6)
-            budget = max(seconds_target - total_s, 5.0)
-            limit = int(min(rate * budget, 50_000_000_000))
-            t = time.perf_counter()
-            cnt, _, stopped = O.match_partial(q.labels, q.adjm(), off, adj, lab, threads=threads, limit=limit)
-            dt = time.perf_counter() - t
-            total_emb += cnt
-            total_s += dt
-            parts.append(f"{name}: {'first ' if stopped else 'all '}{cnt} embeddings")
-            if not stopped:
-                checks[name] = cnt
-    # one thread on the sparse query, bounded to about an eighth of the budget
-    name, q = queries[-1]
-    rate1 = total_emb / total_s / threads
-    limit1 = max(1_000_000, int(rate1 * seconds_target / 8))
-    t = time.perf_counter()
-    cnt1, _, stopped1 = O.match_partial(q.labels, q.adjm(), off, adj, lab, threads=1, limit=limit1)
-    dt1 = time.perf_counter() - t
-    return {"value": total_emb / total_s, "unit": "embeddings/s", "cores": threads, "kind": "port",
+    q = dict(queries)["sparse"]
+
+    def run(limit, th):
+        t = time.perf_counter()
+        cnt, _, stopped = O.match_partial(q.labels, q.adjm(), off, adj, lab, threads=th, limit=limit)
+        return cnt, time.perf_counter() - t, stopped
+
+    cnt, dt, _ = run(20_000_000, threads)
+    limit = int(max(1e6, min(cnt / max(dt, 1e-6) * seconds_target, 5e10)))
+    rates = []
+    for i in range(6):
+        cnt, dt, stopped = run(limit, threads)
+        if i:
+            rates.append(cnt / dt)
+    limit1 = max(1_000_000, limit // 16)
+    rates1 = []
+    for i in range(6):
+        cnt1, dt1, _ = run(limit1, 1)
+        if i:
+            rates1.append(cnt1 / dt1)
+    checks = {}
+    qd = dict(queries)["dense"]
+    dense, _ = O.match(qd.labels, qd.adjm(), off, adj, lab, threads=threads)
+    checks["dense"] = dense
+    return {"value": round(float(np.median(rates)), 1), "unit": "embeddings/s", "cores": threads, "kind": "port",
             "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
-            "single_thread_value": cnt1 / max(dt1, 1e-9),
-            "single_thread_sample": f"{name}: {'first ' if stopped1 else 'all '}{cnt1} embeddings in {dt1:.1f} s",
-            "sample": "oracle backtracking (oracle/oracle.c), roots handed out dynamically to "
-                      f"{threads} threads, {total_s:.1f} s: " + "; ".join(parts)}, checks
+            "single_thread_value": round(float(np.median(rates1)), 1),
+            "runs": [round(r, 1) for r in rates], "single_thread_runs": [round(r, 1) for r in rates1],
+            "sample": f"oracle backtracking (oracle/oracle.c) on the C2 sparse query, the search stopped after "
+                      f"{limit} embeddings ({limit1} on one thread); roots handed out dynamically to "
+                      f"{threads} threads; one warm-up, then the median of 5 runs"}, checks
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "ingest"])
-    ap.add_argument("--qps", type=float, default=0.0, help="c5: arrival rate (0: the workload's)")
-    ap.add_argument("--requests", type=int, default=0, help="c5: stream length (0: the workload's)")
-    ap.add_argument("--workers", type=int, default=4, help="c5: server worker contexts")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--ingest-c5", action="store_true",
-                    help="ingest: also time the C5 graph (R-MAT 2^24 V / 200M E, a 4 GB text file)")
-    ap.add_argument("--alg", type=int, default=1)
-    ap.add_argument("--copy-only", type=int, default=0,
-                    help="only run the STREAM copy kernel over this many bytes (counter calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU: the hot path is single-device; "
-                                   "--gpus N>1 is not supported"}))
-        sys.exit(2)
-
-    if args.workload == "ingest":
-        return bench_ingest(args)
-
+def run_c2(args, with_cpu=True):
     import numpy as np
     import fsig_amd as F
     from fsig_amd import workloads as W
-
-    if args.copy_only:
-        ctx = F.Context(0, 2 * args.copy_only + (64 << 20))
-        gbps = ctx.stream_copy(args.copy_only, 5)
-        print(json.dumps({"stream_copy_bytes": args.copy_only, "iters": 6, "GB_per_s": round(gbps, 1)}))
-        return
-
-    if args.workload == "c4":
-        return bench_c4(args)
-    if args.workload == "c5":
-        return bench_c5(args)
-    if args.workload == "c3":
-        return bench_c3(args)
-
     t0 = time.perf_counter()
     g = W.c2_graph()
     queries = W.c2_queries(g)
-    log(f"graph {g.n} V {g.nnz // 2} E {g.num_labels} labels, generated in {time.perf_counter() - t0:.2f}s")
+    log(f"C2 graph {g.n} V {g.nnz // 2} E {g.num_labels} labels in {time.perf_counter() - t0:.2f}s")
     ctx = F.Context(0, 64 << 30)
     dg = ctx.upload(g, pcsr=True)
-    counts = {}
-    for name, q in queries:
-        counts[name] = F.match(dg, q, args.alg, F.MODE_COUNT).count
-    log("counts", counts)
+    counts = {name: F.match(dg, q, args.alg, F.MODE_COUNT).count for name, q in queries}
+    log("C2 counts", counts)
 
     def step(lat):
         n = 0
@@ -186,9 +177,6 @@ def main():
     elapsed = time.perf_counter() - t
     stats = {F.KERNEL_FAMILIES[f]: ctx.kernel_stats(f) for f in range(len(F.KERNEL_FAMILIES))}
     ctx.set_profiling(False)
-    ms_per_step = elapsed / args.steps * 1e3
-    value = emb / elapsed
-
     kernels = {}
     for fam, s in stats.items():
         if s["launches"]:
@@ -199,193 +187,283 @@ def main():
     d = stats[dom]
     achieved = d["alg_bytes"] / (d["ms"] / 1e3) / 1e9
     kname = KERNEL_OF_FAMILY.get(dom, dom)
-    roofline = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
+    roofline = {"bound": "hbm", "kernel": kname, "workload": "C2", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(kname),
-                "avg_launch_ms": round(d["ms"] / d["launches"], 4),
+                "traffic": measured_traffic(kname), "avg_launch_ms": round(d["ms"] / d["launches"], 4),
                 "alg_bytes_per_launch": int(d["alg_bytes"] / d["launches"])}
-
-    # alg 2 (GSI path) on the same queries, outside the timed region
     gsi = {}
-    if args.alg == 1:
+    if args.alg == 1:  # alg 2 (GSI path) on the same queries, outside the timed region
         t = time.perf_counter()
         n2 = 0
-        for _, q in queries:
+        for name, q in queries:
             c2 = F.match(dg, q, F.ALG_GSI, F.MODE_COUNT).count
+            assert c2 == counts[name], "alg 2 disagrees with alg 1"
             n2 += c2
-            assert c2 == counts[_], "alg 2 disagrees with alg 1"
         gsi = {"alg2_embeddings_per_s": round(n2 / (time.perf_counter() - t), 1)}
-
     stream_gbps = ctx.stream_copy(1 << 30, 20)
     lat_ms = np.array(lat) * 1e3
-    out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "embeddings/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-        "config": {"workload": "C2 dblp-shaped: Chung-Lu gamma 2.3, 317,080 V, 1,049,866 E, 16 Zipf(1.2) labels; "
-                               "8-vertex dense + sparse random-walk queries; alg 1, count mode",
-                   "graph_seed": W.C2_GRAPH["graph_seed"], "label_seed": W.C2_GRAPH["label_seed"],
-                   "queries": {n: {"edges": q.num_edges, "seed": W.C2_QUERIES[n]["seed"], "embeddings": counts[n]}
-                               for n, q in queries},
-                   "parallelism": "1 GPU"},
-        "queries_per_s": round(len(lat) / elapsed, 2), "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
-        "p99_ms": round(float(np.percentile(lat_ms, 99)), 3), "hbm_frac": roofline["frac"],
-        "roofline": roofline, "stream_copy_GB_per_s": round(stream_gbps, 1), "kernels": kernels, **gsi,
-    }
-    if not args.no_cpu:
-        log("cpu baseline ...")
+    out = {"embeddings_per_s": round(emb / elapsed, 1), "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+           "steps": args.steps, "queries_per_s": round(len(lat) / elapsed, 2), "p50_ms": pct(lat_ms, 50),
+           "p99_ms": pct(lat_ms, 99), "workload": "C2 dblp-shaped: Chung-Lu gamma 2.3, 317,080 V, 1,049,866 E, "
+                                                 "16 Zipf(1.2) labels; 8-vertex dense + sparse random-walk queries; "
+                                                 "alg 1, count mode (exact)",
+           "queries": {n: {"edges": q.num_edges, "seed": W.C2_QUERIES[n]["seed"], "embeddings": counts[n]}
+                       for n, q in queries},
+           "kernels": kernels, "stream_copy_GB_per_s": round(stream_gbps, 1), **gsi}
+    cb = None
+    if with_cpu:
+        log("C2 cpu baseline ...")
         cb, checks = cpu_baseline(g, queries)
         for n, c in checks.items():
             assert c == counts[n], f"oracle count {c} != engine count {counts[n]} for {n}"
-        out["cpu_baseline"] = cb
         out["cpu_checked_counts"] = checks
-    print(json.dumps(out), flush=True)
     dg.free()
     ctx.close()
+    return out, roofline, cb
 
 
-def bench_c4(args):
-    """C4: 256 random-walk queries (k uniform in 4..16, half dense, half
-    sparse) on a 1M-vertex / 10M-edge Chung-Lu graph with 32 Zipf(1.2)
-    labels.  The batched engine (one fsig_match_batch call: batched setup +
-    one persistent, graph-launched join) against the plain loop of one
-    fsig_match per query on the same GPU."""
+# ---------------------------------------------------------------------------
+# C4: one bounded batch against the plain loop
+def run_c4(args, steps):
     import numpy as np
     import fsig_amd as F
     from fsig_amd import workloads as W
+    p = W.C4_QUERIES
     t0 = time.perf_counter()
     g = W.c4_graph()
     ctx = F.Context(0, 96 << 30)
     dg = ctx.upload(g)
-    queries = W.c4_queries(g, dg, progress=log)
-    log(f"c4 graph {g.n} V {g.nnz // 2} E, {len(queries)} queries, {time.perf_counter() - t0:.1f}s")
-    # plain loop (one warm-up pass, one timed pass)
-    lat, plain_counts = [], []
-    for q in queries:
-        F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT)
+    info = {}
+    qs = W.c4_queries(g, info)
+    log(f"C4 graph {g.n} V {g.nnz // 2} E, {len(qs)} queries ({info['no_walk']} draws without a walk) in "
+        f"{time.perf_counter() - t0:.1f}s")
+    B, WL = p["bound"], p["work_limit"]
+    ctx.set_work_limit(WL)
+
+    def plain_pass():
+        res, lat = [], []
+        for q in qs:
+            t = time.perf_counter()
+            try:
+                res.append(F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT, max_embeddings=B).count)
+            except F.FsigError as e:
+                if e.code != F.FSIG_ELIMIT:
+                    raise
+                res.append(None)
+            lat.append(time.perf_counter() - t)
+        return res, lat
+
+    plain_pass()  # warm-up
+    ctx.synchronize()
+    plain, lat = plain_pass()
+    adm = [i for i, c in enumerate(plain) if c is not None]
+    plain_s = sum(lat[i] for i in adm)
+    aq = [qs[i] for i in adm]
+    want = [plain[i] for i in adm]
+    log(f"C4 plain: {len(adm)} admitted in {plain_s * 1e3:.0f} ms, {len(qs) - len(adm)} refused")
+    counts, _, st = F.match_batch(dg, aq, bounds=B, status=True)  # warm-up, and the check
+    assert not st.any(), "a query the plain engine admitted was refused in the batch"
+    assert counts.tolist() == want, "batched counts differ from the plain loop"
     ctx.synchronize()
     t = time.perf_counter()
-    for q in queries:
-        t1 = time.perf_counter()
-        plain_counts.append(F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT).count)
-        lat.append(time.perf_counter() - t1)
-    plain_s = time.perf_counter() - t
-    log(f"plain loop {plain_s * 1e3:.1f} ms, {sum(plain_counts)} embeddings")
-    for _ in range(max(1, args.warmup)):
-        counts, _ = F.match_batch(dg, queries)
-    assert counts.tolist() == plain_counts, "batched counts differ from the plain loop"
-    ctx.set_profiling(True)
-    ctx.reset_stats()
+    for _ in range(steps):
+        counts, _ = F.match_batch(dg, aq, bounds=B)
     ctx.synchronize()
-    t = time.perf_counter()
-    join_ms = []
-    for _ in range(args.steps):
-        counts, jm = F.match_batch(dg, queries)
-        join_ms.append(jm)
-    ctx.synchronize()
-    batch_s = (time.perf_counter() - t) / args.steps
-    st = ctx.kernel_stats(F.K_BATCH)
-    lanes, tile, engine = F.last_batch_tune(ctx)
-    lat_ms = np.array(lat) * 1e3
-    emb = int(sum(plain_counts))
-    out = {"metric": METRIC, "value": round(len(queries) / batch_s, 2), "unit": "queries/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(batch_s * 1e3, 3),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-           "config": {"workload": "C4 batched: Chung-Lu 1M V / 10M E, 32 Zipf(1.2) labels; 256 random-walk queries, "
-                                  "k uniform in 4..16, half dense half sparse; one fsig_match_batch per step",
-                      "embeddings_per_step": emb, "parallelism": "1 GPU"},
-           "plain_queries_per_s": round(len(queries) / plain_s, 2),
-           "batched_over_plain": round(plain_s / batch_s, 2),
-           "batched_embeddings_per_s": round(emb / batch_s, 1),
-           "plain_p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
-           "plain_p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
-           "persistent_join_ms": round(float(np.mean(join_ms)), 3),
-           "persistent_join_alg_GB_per_s": round(st["alg_bytes"] / (st["ms"] / 1e3) / 1e9, 1) if st["ms"] else None,
-           "autotuned": {"engine": engine, "lanes": lanes, "lds_tile_words": tile}}
-    print(json.dumps(out), flush=True)
+    batch_s = (time.perf_counter() - t) / steps
+    assert counts.tolist() == want
+    ctx.set_work_limit(0)
+    out = {"workload": "C4 batched: Chung-Lu 1M V / 10M E, 32 Zipf(1.2) labels; the first 256 random-walk "
+                       "queries, k uniform in 4..16, dense and sparse alternating, each counted to a bound of 1e8; "
+                       "one fsig_match_batch_bounded call per step",
+           "batched_queries_per_s": round(len(aq) / batch_s, 2), "plain_queries_per_s": round(len(aq) / plain_s, 2),
+           "batched_over_plain": round(plain_s / batch_s, 3), "ms_per_batch": round(batch_s * 1e3, 3),
+           "steps": steps, "queries": len(qs), "admitted": len(adm), "refused_by_work_limit": len(qs) - len(adm),
+           "exact_below_bound": int(sum(1 for c in want if c < B)), "at_bound": int(sum(1 for c in want if c == B)),
+           "bound": B, "work_limit": WL, "draws_without_walk": info["no_walk"],
+           "embeddings_per_step": int(sum(want)), "plain_p50_ms": pct(np.array(lat) * 1e3, 50),
+           "plain_p99_ms": pct(np.array(lat) * 1e3, 99)}
+    dg.free()
+    ctx.close()
+    return out
 
 
-def bench_c5(args):
-    """C5: a Poisson stream of random-walk count queries (k log-uniform in
-    4..32) on an R-MAT 2^24-vertex / 200M-edge graph with 64 uniform labels,
-    resident in HBM.  The served loop (continuous batching over worker
-    streams, admission by projected bytes) against one-at-a-time plain
-    launches of the same stream in arrival order; latency is arrival to
-    completion in both."""
+# ---------------------------------------------------------------------------
+# C5: the served stream (headline)
+def run_c5(args, steps, warmup):
     import threading
     import numpy as np
     import fsig_amd as F
     from fsig_amd import workloads as W
+    p = W.C5_STREAM
     t0 = time.perf_counter()
     g = W.c5_graph()
-    log(f"c5 graph {g.n} V {g.nnz // 2} E in {time.perf_counter() - t0:.1f}s")
+    log(f"C5 graph {g.n} V {g.nnz // 2} E in {time.perf_counter() - t0:.1f}s")
     ctx = F.Context(0, 32 << 30)
     dg = ctx.upload(g)
-    t0 = time.perf_counter()
-    stream = W.c5_stream(g, dg, args.requests, args.qps, progress=log)
+    info = {}
+    stream = W.c5_stream(g, args.requests, args.qps, info=info)
     arrivals = np.array([a for a, _ in stream])
     qs = [q for _, q in stream]
-    log(f"{len(qs)} requests drawn in {time.perf_counter() - t0:.1f}s, offered {len(qs) / arrivals[-1]:.0f} q/s")
-    # one-at-a-time plain launches, in arrival order
-    for q in qs[:8]:
-        F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT)
+    B, WL = p["bound"], p["work_limit"]
+    log(f"C5 {len(qs)} requests ({info['no_walk']} draws without a walk), offered "
+        f"{len(qs) / max(arrivals[-1], 1e-9):.0f} q/s")
+    # one-at-a-time plain launches in arrival order (after a warm-up pass)
+    ctx.set_work_limit(WL)
+
+    def plain_one(q):
+        try:
+            return F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT, max_embeddings=B).count
+        except F.FsigError as e:
+            if e.code != F.FSIG_ELIMIT:
+                raise
+            return None
+
+    for q in qs:
+        plain_one(q)
     ctx.synchronize()
-    plain_counts, plain_lat = [], []
+    plain, plain_lat = [], []
     start = time.perf_counter()
     for a, q in zip(arrivals, qs):
         now = time.perf_counter() - start
         if now < a:
             time.sleep(a - now)
-        plain_counts.append(F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT).count)
+        plain.append(plain_one(q))
         plain_lat.append((time.perf_counter() - start - a) * 1e3)
     plain_s = time.perf_counter() - start
-    log(f"plain: {plain_s * 1e3:.0f} ms, p50 {np.percentile(plain_lat, 50):.1f} ms")
-    budget = 128 << 30
-    srv = F.Server(dg, workers=args.workers, hbm_budget=budget, max_batch=64)
-    # warm-up pass through the server (first-launch costs)
-    for t in [srv.submit(q) for q in qs[:32]]:
-        srv.wait(t)
-    srv.close()
-    srv = F.Server(dg, workers=args.workers, hbm_budget=budget, max_batch=64)
-    tickets = [None] * len(qs)
+    ctx.set_work_limit(0)
+    log(f"C5 plain: {plain_s * 1e3:.0f} ms, p50 {np.percentile(plain_lat, 50):.1f} ms")
+    srv = F.Server(dg, workers=args.workers, hbm_budget=128 << 30, max_batch=64)
+    srv.set_work_limit(WL)
 
-    def feed(start):
-        for i, (a, q) in enumerate(zip(arrivals, qs)):
-            now = time.perf_counter() - start
-            if now < a:
-                time.sleep(a - now)
-            tickets[i] = srv.submit(q)
+    def serve_once():
+        tickets = [None] * len(qs)
 
-    start = time.perf_counter()
-    th = threading.Thread(target=feed, args=(start,))
-    th.start()
-    th.join()
-    res = [srv.wait(t) for t in tickets]
-    served_s = time.perf_counter() - start
+        def feed(t_start):
+            for i, (a, q) in enumerate(zip(arrivals, qs)):
+                now = time.perf_counter() - t_start
+                if now < a:
+                    time.sleep(a - now)
+                tickets[i] = srv.submit(q, B)
+
+        t_start = time.perf_counter()
+        th = threading.Thread(target=feed, args=(t_start,))
+        th.start()
+        th.join()
+        res = []
+        for t in tickets:
+            try:
+                res.append(srv.wait(t))
+            except F.FsigError as e:
+                if e.code != F.FSIG_ELIMIT:
+                    raise
+                res.append((None, None))
+        return res, time.perf_counter() - t_start
+
+    for _ in range(warmup):
+        serve_once()
+    st0 = srv.stats()
+    served_s, lat_all, served = 0.0, [], None
+    for _ in range(steps):
+        res, dt = serve_once()
+        served_s += dt
+        lat_all += [l for c, l in res if c is not None]
+        got = [c for c, _ in res]
+        for a, b in zip(got, plain):  # requests both engines answered agree
+            assert a is None or b is None or a == b, "served count differs from the plain launch"
+        served = got
     st = srv.stats()
     srv.close()
-    served_counts = [c for c, _ in res]
-    served_lat = np.array([l for _, l in res])
-    assert served_counts == plain_counts, "served counts differ from the plain launches"
-    plain_lat = np.array(plain_lat)
-    out = {"metric": METRIC, "value": round(len(qs) / served_s, 2), "unit": "queries/s", "n_gpus": 1,
-           "steps": 1, "warmup": 1, "ms_per_step": round(served_s * 1e3, 3), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-           "config": {"workload": f"C5 served: R-MAT 2^24 V / {g.nnz // 2} E, 64 uniform labels, resident; "
-                                  f"{len(qs)} Poisson requests at {len(qs) / arrivals[-1]:.0f} q/s offered, "
-                                  "k = round(exp(U(ln 4, ln 32))) conditioned on k <= 5 (larger walk queries "
-                                  "exceed 1e8 embeddings / 2e7 partial rows on this graph), count mode",
-                      "workers": args.workers, "hbm_budget_GiB": budget >> 30, "max_batch": 64,
-                      "parallelism": "1 GPU"},
-           "served_p50_ms": round(float(np.percentile(served_lat, 50)), 3),
-           "served_p99_ms": round(float(np.percentile(served_lat, 99)), 3),
-           "plain_queries_per_s": round(len(qs) / plain_s, 2),
-           "plain_p50_ms": round(float(np.percentile(plain_lat, 50)), 3),
-           "plain_p99_ms": round(float(np.percentile(plain_lat, 99)), 3),
-           "served_over_plain": round(plain_s / served_s, 2),
+    nreq = len(qs) * steps
+    out = {"workload": f"C5 served: R-MAT 2^24 V / {g.nnz // 2} E, 64 uniform labels, resident; "
+                       f"{len(qs)} Poisson requests offered at {len(qs) / max(arrivals[-1], 1e-9):.0f} q/s, "
+                       "k = round(exp(U(ln 4, ln 32))), every drawn query kept, each counted to a bound of 1e8; "
+                       "one stream served per step",
+           "served_queries_per_s": round(nreq / served_s, 2), "ms_per_step": round(served_s / steps * 1e3, 3),
+           "served_p50_ms": pct(lat_all, 50), "served_p99_ms": pct(lat_all, 99),
+           "plain_queries_per_s": round(len(qs) / plain_s, 2), "plain_p50_ms": pct(plain_lat, 50),
+           "plain_p99_ms": pct(plain_lat, 99), "served_over_plain": round((plain_s * steps) / served_s, 3),
+           "requests": len(qs), "refused_by_work_limit": {"plain": sum(1 for c in plain if c is None),
+                                                         "served": sum(1 for c in served if c is None)},
+           "exact_below_bound": sum(1 for c in plain if c is not None and c < B),
+           "at_bound": sum(1 for c in plain if c == B), "bound": B, "work_limit": WL,
+           "draws_without_walk": info["no_walk"],
            "k_histogram": {str(k): int(sum(1 for q in qs if q.k == k)) for k in sorted({q.k for q in qs})},
-           "embeddings": int(sum(plain_counts)),
-           "server": {k: (round(v, 1) if isinstance(v, float) else v) for k, v in st.items()}}
+           "workers": args.workers, "hbm_budget_GiB": 128, "max_batch": 64,
+           "server": {k: (round(v - st0[k], 1) if isinstance(v, float) else v - st0[k])
+                      for k, v in st.items() if k not in ("max_concurrent", "max_batch_seen", "wall_ms")}}
+    dg.free()
+    ctx.close()
+    return out
+
+
+def line(value, unit, steps, warmup, ms_per_step, config, **rest):
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u32", "data": "synthetic", "config": config, **rest}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="all", choices=["all", "c2", "c3", "c4", "c5", "ingest"])
+    ap.add_argument("--qps", type=float, default=0.0, help="c5: arrival rate (0: the workload's)")
+    ap.add_argument("--requests", type=int, default=0, help="c5: stream length (0: the workload's)")
+    ap.add_argument("--workers", type=int, default=4, help="c5: server worker contexts")
+    ap.add_argument("--c4-steps", type=int, default=5, help="all: timed batches of C4")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--ingest-c5", action="store_true",
+                    help="ingest: also time the C5 graph (R-MAT 2^24 V / 200M E, a 4 GB text file)")
+    ap.add_argument("--alg", type=int, default=1)
+    ap.add_argument("--copy-only", type=int, default=0,
+                    help="only run the STREAM copy kernel over this many bytes (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU: the hot path is single-device; "
+                                   "--gpus N>1 is not supported"}))
+        sys.exit(2)
+    if args.workload == "ingest":
+        return bench_ingest(args)
+    import fsig_amd as F
+    if args.copy_only:
+        ctx = F.Context(0, 2 * args.copy_only + (64 << 20))
+        gbps = ctx.stream_copy(args.copy_only, 5)
+        print(json.dumps({"stream_copy_bytes": args.copy_only, "iters": 6, "GB_per_s": round(gbps, 1)}))
+        return
+    if args.workload == "c3":
+        return bench_c3(args)
+    parallel = "1 GPU"
+    if args.workload == "c2":
+        c2, roof, cb = run_c2(args, with_cpu=not args.no_cpu)
+        out = line(c2["embeddings_per_s"], "embeddings/s", args.steps, args.warmup, c2["ms_per_step"],
+                   {"workload": c2["workload"], "parallelism": parallel}, roofline=roof, c2=c2)
+        if cb:
+            out["cpu_baseline"] = cb
+        print(json.dumps(out), flush=True)
+        return
+    if args.workload == "c4":
+        c4 = run_c4(args, args.steps)
+        print(json.dumps(line(c4["batched_queries_per_s"], "queries/s", args.steps, 1, c4["ms_per_batch"],
+                              {"workload": c4["workload"], "parallelism": parallel}, c4=c4)), flush=True)
+        return
+    if args.workload == "c5":
+        c5 = run_c5(args, args.steps, args.warmup)
+        print(json.dumps(line(c5["served_queries_per_s"], "queries/s", args.steps, args.warmup, c5["ms_per_step"],
+                              {"workload": c5["workload"], "parallelism": parallel}, c5=c5)), flush=True)
+        return
+    # all: C2 (roofline, CPU baseline), C4 (batched vs plain), C5 (headline)
+    c2, roof, cb = run_c2(args, with_cpu=not args.no_cpu)
+    c4 = run_c4(args, max(1, min(args.c4_steps, args.steps)))
+    c5 = run_c5(args, args.steps, args.warmup)
+    out = line(c5["served_queries_per_s"], "queries/s", args.steps, args.warmup, c5["ms_per_step"],
+               {"workload": c5["workload"], "parallelism": parallel},
+               served_p50_ms=c5["served_p50_ms"], served_p99_ms=c5["served_p99_ms"],
+               served_over_plain=c5["served_over_plain"], batched_over_plain=c4["batched_over_plain"],
+               embeddings_per_s=c2["embeddings_per_s"], hbm_frac=roof["frac"], roofline=roof,
+               c5=c5, c4=c4, c2=c2)
+    if cb:
+        out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 
 

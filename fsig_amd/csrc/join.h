@@ -667,7 +667,9 @@ class JoinDriver {
  public:
   using Src = typename Provider::SrcType;
   JoinDriver(fsig_ctx* c, Provider& pv, uint32_t k, uint32_t G, int mode, uint64_t max_emb, LeafSink* sink)
-      : c_(c), pv_(pv), k_(k), G_(G), mode_(mode), max_emb_(max_emb), sink_(sink) {}
+      : c_(c), pv_(pv), k_(k), G_(G), mode_(mode), max_emb_(max_emb), sink_(sink) {
+    for (uint32_t p = 0; p <= MAXQ; p++) chunk_[p] = window_[p] = kGrow0;
+  }
 
   uint64_t total() const { return total_; }
 
@@ -679,16 +681,26 @@ class JoinDriver {
   bool bounded() const { return max_emb_ != 0; }
   uint64_t remaining() const { return bounded() ? max_emb_ - total_ : ~0ull; }
 
-  // A bounded match expands its parent rows in chunks that double from 64K
-  // rows, and stops as soon as the bound is reached (an unbounded one takes
-  // every row at once).
+  // A bounded match takes the parent rows of a level in chunks, and the
+  // children of a chunk in windows, whose sizes start at kGrow0 rows and
+  // double each time the level takes one (for the whole match, so a level
+  // reached from many windows above does not start small every time); it
+  // stops as soon as the bound is reached, so most children of a huge query
+  // are never counted or written.  An unbounded match takes every row at once.
+  static constexpr uint64_t kGrow0 = 1ull << 9;
+  static constexpr uint64_t kGrowMax = 1ull << 28;
+  uint64_t chunk_[MAXQ + 1];
+  uint64_t window_[MAXQ + 1];
+
   int expand(uint32_t p, const uint32_t* rows, uint64_t R) {
     if (R == 0) return FSIG_OK;
     if (p == k_ || !bounded()) return expand_rows(p, rows, R);
-    uint64_t chunk = 1ull << 16;
-    for (uint64_t r0 = 0; r0 < R && remaining() > 0; r0 += chunk, chunk *= 2) {
-      const int rc = expand_rows(p, rows + r0 * p, std::min(chunk, R - r0));
+    for (uint64_t r0 = 0; r0 < R && remaining() > 0;) {
+      const uint64_t n = std::min(chunk_[p], R - r0);
+      chunk_[p] = std::min(chunk_[p] * 2, kGrowMax);
+      const int rc = expand_rows(p, rows + r0 * p, n);
       if (rc) return rc;
+      r0 += n;
     }
     return FSIG_OK;
   }
@@ -712,10 +724,8 @@ class JoinDriver {
     int rc;
     bool fused = false;
     if constexpr (std::is_same<Src, CandEdgeSrc>::value)
-      // (a work limit needs the next-to-last level's size before it is
-      // expanded, which only the level-by-level path knows)
-      fused = mode_ == FSIG_MODE_COUNT && p + 2 == k_ && c_->tune.fused_tail == 0 && c_->work_limit == 0 &&
-              tail_applies(pv_.level(p + 1));
+      // (under a work limit the tail's children count after its pass)
+      fused = mode_ == FSIG_MODE_COUNT && p + 2 == k_ && c_->tune.fused_tail == 0 && tail_applies(pv_.level(p + 1));
     if (last && mode_ == FSIG_MODE_COUNT) {
       FSIG_HIP(c_, hipMemsetAsync(res, 0, sizeof(unsigned long long), c_->stream));
       JoinIO io{rows, 0, R, nullptr, nullptr, nullptr, 0, 0, res, family_counters(c_, FSIG_K_JOIN_COUNT)};
@@ -749,23 +759,37 @@ class JoinDriver {
     if (T == 0) { c_->arena.release(mk); return FSIG_OK; }
     if (last) T = std::min(T, remaining());
     const uint64_t limit = c_->work_limit;
-    if (limit && !last && rows_ + T > limit)
-      return set_err(c_, FSIG_ELIMIT, "join: work limit exceeded");
-    rows_ += last ? 0 : T;
-    // Children per window: half of the free arena, each child costing its
-    // row plus the count/offset words it needs when expanded in turn.
+    // Children per window: at most half of the free arena, each child
+    // costing its row plus the count/offset words it needs when expanded in
+    // turn.  Unbounded: balanced windows, all children counted as produced
+    // now.  Bounded: windows growing per level from kGrow0, each counted as
+    // it is produced.
     const uint64_t per_child = 4ull * (p + 1) + 16;
     uint64_t cap = c_->arena.avail() / 2 / per_child;
     if (cap == 0) return set_err(c_, FSIG_ENOMEM, "join: arena exhausted for child rows");
-    const uint64_t nwin = (T + cap - 1) / cap;
-    const uint64_t wcap = (T + nwin - 1) / nwin;  // balanced windows
-    unsigned long long* dwin = nwin > 1 ? (unsigned long long*)c_->arena.alloc(16) : nullptr;
-    if (nwin > 1 && !dwin) return set_err(c_, FSIG_ENOMEM, "join: arena exhausted for windows");
-    for (uint64_t w = 0; w < nwin && remaining() > 0; w++) {
-      const uint64_t lo = w * wcap, hi = std::min(T, lo + wcap);
-      if (lo >= hi) break;
+    const bool grow = bounded();
+    uint64_t wsize;
+    if (grow) {
+      wsize = std::min(cap, window_[p]);
+    } else {
+      if (limit && !last && rows_ + T > limit) return set_err(c_, FSIG_ELIMIT, "join: work limit exceeded");
+      rows_ += last ? 0 : T;
+      const uint64_t nwin = (T + cap - 1) / cap;
+      wsize = (T + nwin - 1) / nwin;  // balanced windows
+    }
+    const bool split = T > wsize;
+    unsigned long long* dwin = split ? (unsigned long long*)c_->arena.alloc(16) : nullptr;
+    if (split && !dwin) return set_err(c_, FSIG_ENOMEM, "join: arena exhausted for windows");
+    for (uint64_t lo = 0; lo < T && remaining() > 0;) {
+      const uint64_t hi = std::min(T, lo + wsize);
+      if (grow) {
+        if (limit && !last && rows_ + (hi - lo) > limit) return set_err(c_, FSIG_ELIMIT, "join: work limit exceeded");
+        rows_ += last ? 0 : hi - lo;
+        window_[p] = std::min(window_[p] * 2, kGrowMax);
+        wsize = std::min(cap, window_[p]);
+      }
       uint64_t win[2] = {0, R};
-      if (nwin > 1) {  // parent rows of this window, found on the device
+      if (split) {  // parent rows of this window, found on the device
         k_window_rows<<<1, 64, 0, c_->stream>>>(off, R, lo, hi, dwin);
         FSIG_HIP(c_, hipGetLastError());
         if ((rc = read_u64(c_, dwin, win, 2))) return rc;
@@ -782,6 +806,7 @@ class JoinDriver {
       }
       if ((rc = expand(p + 1, child, hi - lo))) return rc;
       c_->arena.release(m2);
+      lo = hi;
     }
     c_->arena.release(mk);
     return FSIG_OK;

@@ -11,16 +11,16 @@ seed names a workload exactly:
       (< 12 edges), both induced subgraphs of seeded random walks.
   C3  unlabeled R-MAT (0.57, 0.19, 0.19), 2^22 vertices, 35M edges; triangle,
       4-cycle, 4-clique and house.
-  C4  Chung-Lu 1M vertices, 10M edges, 32 Zipf(1.2) labels; 256 random-walk
-      queries with k uniform in 4..16, half dense and half sparse.
+  C4  Chung-Lu 1M vertices, 10M edges, 32 Zipf(1.2) labels; the first 256
+      random-walk queries with k uniform in 4..16, half dense and half sparse,
+      each counted to a bound of 1e8 embeddings.
   C5  R-MAT 2^24 vertices, 200M edges, 64 uniform labels; a Poisson stream of
-      random-walk queries with k = round(exp(U(ln 4, ln 32))).
+      random-walk queries with k = round(exp(U(ln 4, ln 32))), each counted
+      to a bound of 1e8 embeddings.
 """
 from __future__ import annotations
 
-import json
 import math
-import os
 from typing import List, Tuple
 
 import numpy as np
@@ -73,34 +73,40 @@ def c4_graph() -> HostGraph:
     return HostGraph.chung_lu(1_000_000, 10_000_000, 2.3, 5).set_labels("zipf", 32, 1.2, 6)
 
 
-# Walks whose query has more than 1e8 embeddings, or whose plain join would
-# hold more than 2e7 partial embeddings, are redrawn (the next seed).
-C4_QUERIES = dict(count=256, kmin=4, kmax=16, seed=4, max_count=10 ** 8, max_rows=2 * 10 ** 7)
+# C4 and C5 keep every drawn walk: no query is redrawn for its size.  Most
+# walk-induced queries on these graphs have astronomically many embeddings
+# (more than 1e9 for about 4 in 5 of the C4 draws), so each query is counted
+# with a bound: its result is min(count, bound), exact below the bound
+# (fsig_match's max_embeddings).  A query whose search produces more partial
+# embeddings than the work limit is refused (FSIG_ELIMIT) and reported.
+C4_QUERIES = dict(count=256, kmin=4, kmax=16, seed=4, bound=10 ** 8, work_limit=2 * 10 ** 9)
 
 
-def c4_queries(g: HostGraph, dg, progress=None) -> List[Query]:
+def c4_queries(g: HostGraph, info: dict = None) -> List[Query]:
+    """The first 256 walk queries of the C4 law: k uniform in 4..16, dense and
+    sparse alternating."""
     p = C4_QUERIES
-    return [q for _, q in walk_queries(g, p["count"], p["kmin"], p["kmax"], p["seed"], dg=dg,
-                                       max_count=p["max_count"], max_rows=p["max_rows"], progress=progress)]
+    return [q for _, q in walk_queries(g, p["count"], p["kmin"], p["kmax"], p["seed"], info=info)]
 
 
 def walk_queries(g: HostGraph, count: int, kmin: int, kmax: int, seed: int, log_uniform: bool = False,
                  max_edges: int = 10 ** 6, dg=None, max_count: int = 0,
                  max_rows: int = 0, progress=None, max_attempts: int = 2000,
-                 alternate_drawn: bool = False, k_cap: int = 0) -> List[Tuple[str, Query]]:
+                 alternate_drawn: bool = False, info: dict = None) -> List[Tuple[str, Query]]:
     """count queries, k uniform (or log-uniform) in [kmin, kmax], alternating
-    dense (average degree >= 3) and sparse; seeds that find no walk with the
-    wanted density are skipped deterministically.  With a device graph `dg`,
-    a query is also rejected when it has more than max_count embeddings or
-    its plain join would build more than max_rows intermediate rows (the
-    engine's bounded count and work limit decide, so the check stops early
-    on huge queries).  With alternate_drawn, dense and sparse alternate over
-    the drawn walks instead of the accepted queries; with k_cap, draws of a
-    larger k are skipped (the law of k conditioned on k <= k_cap)."""
+    dense (average degree >= 3) and sparse; a draw whose walks find no query
+    of the wanted density within max_attempts is skipped deterministically
+    (counted in info["no_walk"]).  With a device graph `dg`, a query is also
+    rejected when it has more than max_count embeddings or its plain join
+    would build more than max_rows intermediate rows (the engine's bounded
+    count and work limit decide, so the check stops early on huge queries);
+    the workloads never pass dg, tests use it to keep oracle checks small.
+    With alternate_drawn, dense and sparse alternate over the drawn walks
+    instead of the accepted queries."""
     rng = np.random.default_rng(seed)
     out = []
     s = seed * 100003
-    draws = 0
+    draws = no_walk = 0
     while len(out) < count:
         draws += 1
         if progress is not None and draws % 25 == 0:
@@ -109,16 +115,15 @@ def walk_queries(g: HostGraph, count: int, kmin: int, kmax: int, seed: int, log_
             k = int(round(math.exp(rng.uniform(math.log(kmin), math.log(kmax)))))
         else:
             k = int(rng.integers(kmin, kmax + 1))
-        if k_cap and k > k_cap:
-            continue  # conditioned on k <= k_cap
         # dense and sparse alternate over accepted queries (an even mix), or
-        # over drawn walks (the mix the bounds let through)
+        # over drawn walks
         dense = (draws if alternate_drawn else len(out)) % 2 == 0
         lo, hi = ((3 * k + 1) // 2, max_edges) if dense else (k - 1, (3 * k - 1) // 2)
         s += 1
         try:
             q = g.walk_query(k, lo, hi, seed=s, bias=2 if dense else 0, max_attempts=max_attempts)
         except FsigError:
+            no_walk += 1
             continue
         if dg is not None and (max_count or max_rows):
             from . import match, ALG_MMATCH, MODE_COUNT, FSIG_ELIMIT
@@ -136,6 +141,8 @@ def walk_queries(g: HostGraph, count: int, kmin: int, kmax: int, seed: int, log_
         out.append(("dense" if dense else "sparse", q))
         if progress is not None and len(out) % 32 == 0:
             progress(f"{len(out)} of {count} queries drawn")
+    if info is not None:
+        info.update(draws=draws, no_walk=no_walk)
     return out
 
 
@@ -143,54 +150,22 @@ def c5_graph(scale: int = 24, m: int = 200_000_000, seed: int = 8) -> HostGraph:
     return HostGraph.rmat(scale, m, 0.57, 0.19, 0.19, seed).set_labels("uniform", 64, seed=9)
 
 
-# C5 request stream: k = round(exp(U(ln 4, ln 32))) conditioned on k <= 5,
-# dense and sparse walks alternating per draw (200 attempts per walk),
-# redrawn above the C4 bounds (more than 1e8 embeddings or 2e7 partial
-# embeddings).  On this R-MAT graph every walk-induced query with k >= 6 that
-# was tried exceeded them (counts of 1e9..1e10 and more), so the stream is
-# the servable part of the law.  Poisson arrivals at `qps` (exponential gaps
-# from their own seed).  Drawing the stream checks every candidate on the
-# device, so a drawn stream is kept in a JSON file: each query with the walk
-# seed that reproduces it.
-C5_STREAM = dict(count=128, kmin=4, kmax=32, k_cap=5, seed=11, arrival_seed=12, qps=1000.0, max_count=10 ** 8,
-                 max_rows=2 * 10 ** 7, max_attempts=200)
-C5_STREAM_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "c5_stream.json")
+# C5 request stream: k = round(exp(U(ln 4, ln 32))), dense and sparse walks
+# alternating per draw (200 attempts per walk), every drawn query kept,
+# counted with a bound and refused over the work limit as C4's; Poisson
+# arrivals at `qps` (exponential gaps from their own seed).
+C5_STREAM = dict(count=128, kmin=4, kmax=32, seed=11, arrival_seed=12, qps=1000.0, bound=10 ** 8,
+                 work_limit=2 * 10 ** 9, max_attempts=200)
 
 
-def _stream_key(p):
-    key = {k: p[k] for k in ("count", "kmin", "kmax", "k_cap", "seed", "max_count", "max_rows", "max_attempts")}
-    key["alternate"] = "drawn"
-    return key
-
-
-def c5_stream(g: HostGraph, dg, count: int = 0, qps: float = 0.0, progress=None,
-              cache: str = C5_STREAM_FILE) -> List[Tuple[float, Query]]:
-    """(arrival time in s from the stream start, query) pairs.  Queries come
-    from `cache` when it holds a stream drawn with the same parameters
-    (otherwise they are drawn, and written there when cache is a path)."""
+def c5_stream(g: HostGraph, count: int = 0, qps: float = 0.0, info: dict = None) -> List[Tuple[float, Query]]:
+    """(arrival time in s from the stream start, query) pairs."""
     p = dict(C5_STREAM)
     if count:
         p["count"] = count
     qps = qps or p["qps"]
-    qs = None
-    if cache and os.path.exists(cache):
-        with open(cache) as f:
-            rec = json.load(f)
-        if rec.get("key") == _stream_key(p):
-            qs = [Query(r["k"], r["labels"], r["edges"]) for r in rec["queries"]]
-    if qs is None:
-        drawn = [q for _, q in walk_queries(g, p["count"], p["kmin"], p["kmax"], p["seed"], log_uniform=True,
-                                            dg=dg, max_count=p["max_count"], max_rows=p["max_rows"],
-                                            progress=progress, max_attempts=p["max_attempts"],
-                                            alternate_drawn=True, k_cap=p["k_cap"])]
-        if cache:
-            os.makedirs(os.path.dirname(cache), exist_ok=True)
-            with open(cache, "w") as f:
-                json.dump({"key": _stream_key(p),
-                           "queries": [{"k": q.k, "labels": [int(x) for x in q.labels],
-                                        "edges": [[int(a), int(b)] for a, b in q.edges.tolist()],
-                                        "walk_seed": int(q.walk_seed)} for q in drawn]}, f)
-        qs = drawn
+    qs = [q for _, q in walk_queries(g, p["count"], p["kmin"], p["kmax"], p["seed"], log_uniform=True,
+                                     max_attempts=p["max_attempts"], alternate_drawn=True, info=info)]
     gaps = np.random.default_rng(p["arrival_seed"]).exponential(1.0 / qps, size=len(qs))
     t = np.cumsum(gaps) - gaps[0]
     return list(zip(t.tolist(), qs))

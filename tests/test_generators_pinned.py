@@ -1,6 +1,5 @@
 """The seeded generators produce the same graphs from build to build: the
-benchmarks' configurations and the committed C5 query stream
-(fsig_amd/data/c5_stream.json, walk seeds on the C5 graph) depend on it.
+benchmarks' configurations and their seeded random-walk queries depend on it.
 SHA-256 of (offsets, adjacency, labels) for R-MAT below and above the size
 where duplicates are removed over key shards on threads (4M edges), and for a
 labeled Chung-Lu graph."""
