@@ -1313,6 +1313,7 @@ struct BatchProvider {
     fill(p, L, S);
     return S;
   }
+  bool chain_tail(uint32_t, ChainTail&) const { return false; }
 };
 
 // bounds (may be null): per query, 0 = exact count, else the count is

@@ -87,6 +87,7 @@ struct GsiProvider {
     }
     return L;
   }
+  bool chain_tail(uint32_t, ChainTail&) const { return false; }
   PcsrSrc src(uint32_t p) const {
     PcsrSrc S;
     std::memset(&S, 0, sizeof(S));

@@ -324,6 +324,9 @@ struct MMatchProvider {
   // pair lists: index by (class a, class b)
   const uint32_t* poff[MAXQ][MAXQ];
   const uint32_t* plst[MAXQ][MAXQ];
+  // data vertex -> candidate index maps of the classes that have one (null:
+  // binary search)
+  const uint32_t* pos[MAXQ];
 
   JoinLevel level(uint32_t p) const {
     JoinLevel L;
@@ -332,6 +335,7 @@ struct MMatchProvider {
     L.p = p;
     L.wcand = cand[P->cls[w]];
     L.wsize = csize[P->cls[w]];
+    L.wpos = pos[P->cls[w]];
     for (uint32_t c = 0; c < p; c++) {
       const uint32_t u = P->order[c];
       if ((P->adjm[w] >> u) & 1u) {
@@ -363,6 +367,15 @@ struct MMatchProvider {
       }
     }
     return S;
+  }
+  // chain tail of the last two levels (set up by run_mmatch when the query
+  // ends that way; chain_p is its parent level)
+  ChainTail chain;
+  uint32_t chain_p = 0xffffffffu;
+  bool chain_tail(uint32_t p, ChainTail& t) const {
+    if (p != chain_p) return false;
+    t = chain;
+    return true;
   }
 };
 
@@ -548,7 +561,26 @@ int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_
   MMatchProvider pv;
   std::memset(&pv, 0, sizeof(pv));
   pv.P = &P;
+  pv.chain_p = 0xffffffffu;
   for (uint32_t a = 0; a < nc; a++) { pv.cand[a] = cur[a]; pv.csize[a] = size[a]; }
+  // A count whose last two levels form a chain (see ChainTail) also needs
+  // the reverse pair (last class -> previous class).
+  const bool chain = mode == FSIG_MODE_COUNT && k >= 3 && c->tune.fused_tail == 0 &&
+                     chain_tail_applies(pv.level(k - 2), pv.level(k - 1));
+  const uint32_t cls_w = P.cls[P.order[k - 1]], cls_p = P.cls[P.order[k >= 2 ? k - 2 : 0]];
+  if (chain && pair_id[cls_w][cls_p] < 0) {
+    pair_id[cls_w][cls_p] = (int)pairs.size();
+    PairDesc pd;
+    pd.bpos = nullptr;
+    pd.acand = cur[cls_w];
+    pd.bcand = cur[cls_p];
+    pd.asize = size[cls_w];
+    pd.bsize = size[cls_p];
+    pd.bbit = cls_p;
+    pd.off_out = nullptr;
+    pd.flat_base = 0;
+    pairs.push_back(pd);
+  }
   double avg_list = 1;
   if (!pairs.empty()) {
     KTimer tc(c, FSIG_K_CEDGE);
@@ -577,6 +609,7 @@ int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_
       }
       if (!fit)
         for (auto& pd : pairs) pd.bpos = nullptr;
+      for (uint32_t b = 0; b < nc && fit; b++) pv.pos[b] = bpos_of[b];  // the join's conflict lookups
     }
     PairDesc* dP = (PairDesc*)A.alloc(sizeof(PairDesc) * np);
     uint32_t* cnt = (uint32_t*)A.alloc(4ull * flat);
@@ -634,6 +667,22 @@ int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_
     // bytes: per candidate row two passes over (id, offset pair) and its
     // adjacency, modelled by the list volume; lists and offsets written once
     tc.done(2ull * flat * 12 + 8ull * total_edges + 4ull * total_edges + 4ull * (flat + np), flat);
+  }
+  if (chain) {
+    // S[v] over the candidates v of the previous position's backward
+    // neighbour: the leaves of a row before conflicts are removed
+    const JoinLevel L1 = pv.level(k - 2);
+    const uint32_t cls_c1 = P.cls[P.order[L1.bcol[0]]];
+    const uint32_t nv = size[cls_c1];
+    unsigned long long* sum = (unsigned long long*)A.alloc(8ull * std::max<uint32_t>(nv, 1));
+    if (!sum) return set_err(c, FSIG_ENOMEM, "join: arena exhausted for the chain tail");
+    KTimer tt(c, FSIG_K_JOIN_TAIL);
+    k_chain_sum<16><<<grid_for(c, nv, FSIG_BLOCK / 16), FSIG_BLOCK, 0, c->stream>>>(
+        pv.poff[cls_c1][cls_p], pv.plst[cls_c1][cls_p], pv.poff[cls_p][cls_w], nv, sum);
+    FSIG_HIP(c, hipGetLastError());
+    tt.done(16ull * nv, 0);  // offsets of each list; its entries and own offsets are charged by the tail
+    pv.chain = ChainTail{sum, pv.poff[cls_w][cls_p], pv.plst[cls_w][cls_p]};
+    pv.chain_p = k - 2;
   }
 
   // ---- join

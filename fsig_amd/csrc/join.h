@@ -42,7 +42,19 @@ struct JoinLevel {
   uint32_t cfcol[MAXQ];
   const uint32_t* cfcand[MAXQ];  // row word -> data vertex for cross conflicts
   const uint32_t* wcand;         // candidate index -> data vertex of w
+  // data vertex -> candidate index of w (n words; meaningful where
+  // wcand[wpos[v]] == v), or null: cand_index then binary-searches wcand
+  const uint32_t* wpos;
 };
+
+constexpr uint32_t kNone = 0xffffffffu;
+
+// Candidate index of data vertex id in w's class, or kNone when id is no
+// candidate of w: one or two gathers through wpos, else a binary search.
+__device__ __forceinline__ uint32_t cand_index(const JoinLevel& L, uint32_t id) {
+  const uint32_t at = L.wpos ? L.wpos[id] : sorted_index(L.wcand, L.wsize, id);
+  return (at < L.wsize && L.wcand[at] == id) ? at : kNone;
+}
 
 struct JoinIO {
   const uint32_t* rows;
@@ -178,13 +190,8 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join(JoinLevel L, Src S, JoinIO 
         bool in = false;
         if (j < L.ncf) {
           uint32_t target = sh_cf[gw][j];
-          bool ok = true;
-          if (!((L.cfsame >> j) & 1u)) {  // data vertex id -> candidate index of w
-            const uint32_t at = sorted_index(L.wcand, L.wsize, target);
-            ok = at < L.wsize && L.wcand[at] == target;
-            target = at;
-          }
-          in = ok && sorted_contains(dl, dlen, target);
+          if (!((L.cfsame >> j) & 1u)) target = cand_index(L, target);  // data vertex id -> candidate index of w
+          in = target != kNone && sorted_contains(dl, dlen, target);
         }
         hit += __popcll(__ballot(in) & gmask);
       }
@@ -270,8 +277,6 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join(JoinLevel L, Src S, JoinIO 
 // counted by the group against all lists; with no shared list the count is
 // |own(x)| minus the parent-column conflicts found in it.  Level p+1 is never
 // written.
-constexpr uint32_t kNone = 0xffffffffu;
-
 template <int G>
 __device__ __forceinline__ uint64_t group_sum_u64(uint64_t v) {
 #pragma unroll
@@ -343,11 +348,7 @@ __device__ __forceinline__ uint64_t tail_row(const JoinLevel& L1, const CandEdge
     const uint32_t c = L2.cfcol[j];
     if (c == p) { ccf = j; sh.cf2[j] = kNone; continue; }
     uint32_t t = group_col<G>(rv, c);
-    if (!((L2.cfsame >> j) & 1u)) {
-      const uint32_t id = L2.cfcand[j][t];
-      const uint32_t at = sorted_index(L2.wcand, L2.wsize, id);
-      t = (at < L2.wsize && L2.wcand[at] == id) ? at : kNone;
-    }
+    if (!((L2.cfsame >> j) & 1u)) t = cand_index(L2, L2.cfcand[j][t]);
     sh.cf2[j] = t;
   }
   uint64_t lbytes = gl == 0 ? 4ull * p + 8ull * (L1.nb + ns) + 4ull * el : 0;
@@ -380,6 +381,26 @@ __device__ __forceinline__ uint64_t tail_row(const JoinLevel& L1, const CandEdge
         isz += (uint32_t)__popcll(__ballot(ok) & gmask);
       }
     }
+    if (ccf == kNone && L1.nb == 1) {
+      // Every child has exactly |I| leaves and the children are the one
+      // list minus the parent-column conflicts found in it: count those
+      // (lanes take one conflict each) instead of walking the children.
+      uint32_t hit = 0;
+      for (uint32_t j0 = 0; j0 < L1.ncf; j0 += G) {
+        const uint32_t j = j0 + gl;
+        bool in = false;
+        if (j < L1.ncf) {
+          uint32_t x = sh.cf1[j];
+          if (!((L1.cfsame >> j) & 1u)) x = cand_index(L1, x);  // a data vertex id
+          in = x != kNone && sorted_contains(dl, dlen, x);
+        }
+        hit += (uint32_t)__popcll(__ballot(in) & gmask);
+      }
+      children = dlen - hit;
+      if (gl == 0) lbytes += 4ull * L1.ncf * (2 + (32 - __builtin_clz(dlen | 1))) - 4ull * dlen;  // list not read
+      bytes += lbytes;
+      return gl == 0 ? children * isz : 0;
+    }
     uint64_t kids = 0, leaves = 0;
     for (uint32_t t0 = 0; t0 < dlen; t0 += G) {
       const uint32_t t = t0 + gl;
@@ -399,11 +420,7 @@ __device__ __forceinline__ uint64_t tail_row(const JoinLevel& L1, const CandEdge
         uint64_t cnt = isz;
         if (ccf != kNone) {  // x's image is a leaf candidate unless some shared list lacks it
           uint32_t cx = x;
-          if (!((L2.cfsame >> ccf) & 1u)) {
-            const uint32_t id = L2.cfcand[ccf][x];
-            const uint32_t at = sorted_index(L2.wcand, L2.wsize, id);
-            cx = (at < L2.wsize && L2.wcand[at] == id) ? at : kNone;
-          }
+          if (!((L2.cfsame >> ccf) & 1u)) cx = cand_index(L2, L2.cfcand[ccf][x]);
           bool in = cx != kNone;
           for (uint32_t i = 0; i < L2.nb && in; i++) in = sorted_contains(S2.lst[i] + sh.s2[i], sh.n2[i], cx);
           if (in) cnt--;
@@ -514,11 +531,15 @@ __device__ __forceinline__ uint64_t tail_row(const JoinLevel& L1, const CandEdge
 // needs it).
 constexpr uint32_t kTailMax = 16;
 
-template <int G>
+// N1: entries of the level-p arrays (MAXQ when a lockstep launch also keeps
+// its plain rows there, kTailMax for the plain tail, whose levels are
+// limited to that many backward lists and conflicts: less LDS per group,
+// more groups per CU).
+template <int G, int N1 = MAXQ>
 struct TailShared {
   static constexpr int GPB = FSIG_BLOCK / G;
   static constexpr uint32_t kTileWords = 4096 / GPB;  // at most 16 KiB of tiles per block
-  uint32_t s1[GPB][MAXQ], n1[GPB][MAXQ], cf1[GPB][MAXQ];  // also the plain rows of a lockstep launch
+  uint32_t s1[GPB][N1], n1[GPB][N1], cf1[GPB][N1];
   uint32_t s2[GPB][kTailMax], n2[GPB][kTailMax], cf2[GPB][kTailMax];
   uint32_t slot[GPB][G];
   __device__ TailLds at(uint32_t gw, uint32_t* tile, uint32_t tw) {
@@ -540,7 +561,7 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join_tail(JoinLevel L1, CandEdge
                                                           CandEdgeSrc S2, TailIO io) {
   constexpr int GPB = FSIG_BLOCK / G;
   constexpr int NR = (MAXQ + G - 1) / G;
-  __shared__ TailShared<G> sh;
+  __shared__ TailShared<G, kTailMax> sh;
   extern __shared__ uint32_t tail_tile[];
   const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
   const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
@@ -607,6 +628,183 @@ inline int launch_join_tail(fsig_ctx* c, uint32_t G, const JoinLevel& L1, const 
     case 16: k_join_tail<16><<<grid, FSIG_BLOCK, lds, c->stream>>>(L1, S1, L2, S2, io); break;
     case 32: k_join_tail<32><<<grid, FSIG_BLOCK, lds, c->stream>>>(L1, S1, L2, S2, io); break;
     default: k_join_tail<64><<<grid, FSIG_BLOCK, lds, c->stream>>>(L1, S1, L2, S2, io); break;
+  }
+  FSIG_HIP(c, hipGetLastError());
+  return FSIG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Chain tail (count mode).  When the leaf w (position p+1) has one backward
+// neighbour, the previous position p, and position p has one backward list
+// (of parent column c1), a parent row's children are C1 = list(phi(c1))
+// minus X1 (the parent columns of p's label that occur in the list), and a
+// child x has |own(x)| leaves minus the parent columns of w's label in
+// own(x) (the conflicts y_j).  Summed over the children:
+//
+//   leaves = S[phi(c1)] - sum_{x in X1} (|own(x)| - #{j : y_j in own(x)})
+//                       - sum_j |list(phi(c1)) ∩ rev(y_j)|
+//
+// where S[v] = sum of |own(x)| over list(v) is computed once per match
+// (k_chain_sum) and rev(y) is the list of p's candidates adjacent to leaf
+// candidate y (the candidate-edge list of the reverse class pair): a row
+// costs a few gathers and one intersection per conflict instead of a walk
+// over its children.  Paths and other tree-shaped queries end this way.
+struct ChainTail {
+  const unsigned long long* sum;  // S[v] for every candidate v of c1's class
+  const uint32_t* rev_off;        // reverse pair (w's class -> p's class)
+  const uint32_t* rev_lst;
+};
+
+// S[v] = sum over x in list(v) (pair c1 -> p) of |own(x)| (pair p -> w); G
+// lanes per candidate v.
+template <int G>
+__global__ void __launch_bounds__(FSIG_BLOCK) k_chain_sum(const uint32_t* __restrict__ off1,
+                                                          const uint32_t* __restrict__ lst1,
+                                                          const uint32_t* __restrict__ own_off, uint32_t nv,
+                                                          unsigned long long* __restrict__ sum) {
+  constexpr int GPB = FSIG_BLOCK / G;
+  const uint32_t gl = threadIdx.x & (G - 1), gw = threadIdx.x / G;
+  for (uint64_t v = (uint64_t)xcd_block() * GPB + gw; v < nv; v += (uint64_t)gridDim.x * GPB) {
+    unsigned long long s = 0;
+    for (uint32_t t = off1[v] + gl; t < off1[v + 1]; t += G) {
+      const uint32_t x = lst1[t];
+      s += own_off[x + 1] - own_off[x];
+    }
+    s = group_sum_u64<G>(s);
+    if (gl == 0) sum[v] = s;
+  }
+}
+
+template <int G>
+__global__ void __launch_bounds__(FSIG_BLOCK) k_join_tail_chain(JoinLevel L1, CandEdgeSrc S1, JoinLevel L2,
+                                                                CandEdgeSrc S2, ChainTail T, TailIO io) {
+  constexpr int GPB = FSIG_BLOCK / G;
+  constexpr int NR = (MAXQ + G - 1) / G;
+  const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
+  const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
+  const uint64_t ngroups = (uint64_t)gridDim.x * GPB;
+  const uint32_t p = L1.p;
+  const uint32_t* own_off = S2.off[0];
+  const uint32_t* own_lst = S2.lst[0];
+  __shared__ uint32_t sh_y[GPB][kTailMax];
+  __shared__ uint32_t sh_x1[GPB][kTailMax];
+  uint32_t* y = sh_y[gw];
+  uint32_t* x1 = sh_x1[gw];
+  unsigned long long my_leaves = 0, my_kids = 0, my_bytes = 0, my_rows = 0;
+  for (uint64_t r = (uint64_t)xcd_block() * GPB + gw; r < io.nrows; r += ngroups) {
+    uint32_t rv[NR];
+    const uint32_t* row = io.rows + r * p;
+#pragma unroll
+    for (int j = 0; j < NR; j++) {
+      const uint32_t col = j * G + gl;
+      rv[j] = col < p ? row[col] : 0u;
+    }
+    const uint32_t v = group_col<G>(rv, L1.bcol[0]);
+    const uint32_t ls = S1.off[0][v], ln = S1.off[0][v + 1] - ls;
+    const uint32_t* lst = S1.lst[0] + ls;
+    int64_t leaves = (int64_t)T.sum[v];
+    uint64_t kids = ln;
+    uint64_t bytes = 4ull * p + 16;
+    // conflicts of the leaf: parent columns of w's label, as candidate
+    // indices of w's class (kNone: not a candidate, nothing to exclude);
+    // every lane of the group computes the same values
+    lds_wave_sync();  // the previous row's reads of y are done
+    for (uint32_t j = 0; j < L2.ncf; j++) {
+      uint32_t t = group_col<G>(rv, L2.cfcol[j]);
+      if (!((L2.cfsame >> j) & 1u)) {
+        t = cand_index(L2, L2.cfcand[j][t]);
+        bytes += 12;
+      }
+      if (gl == 0) y[j] = t;
+    }
+    // X1 candidates: parent columns of p's label as candidate indices of
+    // p's class (a cross-lane read of the row, so in a loop every lane runs)
+    for (uint32_t j = 0; j < L1.ncf; j++) {
+      uint32_t x = group_col<G>(rv, L1.cfcol[j]);
+      if (!((L1.cfsame >> j) & 1u)) x = cand_index(L1, L1.cfcand[j][x]);  // a data vertex id
+      if (gl == 0) x1[j] = x;
+    }
+    lds_wave_sync();
+    // X1: parent columns of p's label found in the list; each removes a
+    // child and its leaves (lanes take one conflict each)
+    for (uint32_t j0 = 0; j0 < L1.ncf; j0 += G) {
+      const uint32_t j = j0 + gl;
+      int64_t sub = 0;
+      bool in = false;
+      uint32_t x = kNone;
+      if (j < L1.ncf) {
+        x = x1[j];
+        in = x != kNone && sorted_contains(lst, ln, x);
+        if (in) {
+          const uint32_t os = own_off[x], on = own_off[x + 1] - os;
+          sub = on;
+          for (uint32_t k2 = 0; k2 < L2.ncf; k2++)
+            if (y[k2] != kNone && sorted_contains(own_lst + os, on, y[k2])) sub--;
+        }
+      }
+      kids -= (uint64_t)__popcll(__ballot(in) & gmask);
+      leaves -= (int64_t)group_sum_u64<G>((uint64_t)sub);
+      if (gl == 0) bytes += 4ull * (G < L1.ncf - j0 ? G : L1.ncf - j0) * (2 + (32 - __builtin_clz(ln | 1)));
+    }
+    // children that hold a conflict in their own list: |list ∩ rev(y_j)|,
+    // the shorter list driving and the other searched
+    for (uint32_t j = 0; j < L2.ncf; j++) {
+      if (y[j] == kNone) continue;
+      const uint32_t rs = T.rev_off[y[j]], rn = T.rev_off[y[j] + 1] - rs;
+      const uint32_t* rl = T.rev_lst + rs;
+      const bool drive_list = ln <= rn;
+      const uint32_t* a = drive_list ? lst : rl;
+      const uint32_t* b = drive_list ? rl : lst;
+      const uint32_t an = drive_list ? ln : rn, bn = drive_list ? rn : ln;
+      uint32_t hit = 0;
+      for (uint32_t t0 = 0; t0 < an; t0 += G) {
+        const uint32_t t = t0 + gl;
+        const bool in = t < an && sorted_contains(b, bn, a[t]);
+        hit += (uint32_t)__popcll(__ballot(in) & gmask);
+      }
+      leaves -= hit;
+      bytes += 8ull + 4ull * an * (1 + (32 - __builtin_clz(bn | 1)));
+    }
+    if (gl == 0) {
+      io.cnt[r] = 0;
+      my_leaves += (unsigned long long)leaves;
+      my_kids += kids;
+      my_bytes += bytes;
+      my_rows += 1;
+    }
+  }
+  my_leaves = wave_sum_u64(my_leaves);
+  my_kids = wave_sum_u64(my_kids);
+  my_bytes = wave_sum_u64(my_bytes);
+  my_rows = wave_sum_u64(my_rows);
+  if (lane == 0) {
+    atomicAdd(&io.res[0], my_leaves);
+    atomicAdd(&io.res[1], my_kids);
+    if (io.stats) {
+      atomicAdd(&io.stats[0], my_bytes);
+      atomicAdd(&io.stats[1], my_rows);
+    }
+  }
+}
+
+// Whether the chain tail applies to levels p (L1) and p+1 (L2).
+inline bool chain_tail_applies(const JoinLevel& L1, const JoinLevel& L2) {
+  return L2.nb == 1 && L2.bcol[0] == L1.p && L1.nb == 1 && L1.ncf <= kTailMax && L2.ncf <= kTailMax;
+}
+
+inline int launch_join_tail_chain(fsig_ctx* c, uint32_t G, const JoinLevel& L1, const CandEdgeSrc& S1,
+                                  const JoinLevel& L2, const CandEdgeSrc& S2, const ChainTail& T,
+                                  const TailIO& io) {
+  if (io.nrows == 0) return FSIG_OK;
+  const uint64_t gpb = FSIG_BLOCK / G;
+  uint64_t blocks = std::min<uint64_t>((io.nrows + gpb - 1) / gpb, (uint64_t)c->num_cus * 16);
+  if (blocks >= 64) blocks &= ~7ull;
+  const uint32_t grid = (uint32_t)std::max<uint64_t>(blocks, 1);
+  switch (G) {
+    case 4: k_join_tail_chain<4><<<grid, FSIG_BLOCK, 0, c->stream>>>(L1, S1, L2, S2, T, io); break;
+    case 8: k_join_tail_chain<8><<<grid, FSIG_BLOCK, 0, c->stream>>>(L1, S1, L2, S2, T, io); break;
+    case 16: k_join_tail_chain<16><<<grid, FSIG_BLOCK, 0, c->stream>>>(L1, S1, L2, S2, T, io); break;
+    default: k_join_tail_chain<32><<<grid, FSIG_BLOCK, 0, c->stream>>>(L1, S1, L2, S2, T, io); break;
   }
   FSIG_HIP(c, hipGetLastError());
   return FSIG_OK;
@@ -725,7 +923,8 @@ class JoinDriver {
     bool fused = false;
     if constexpr (std::is_same<Src, CandEdgeSrc>::value)
       // (under a work limit the tail's children count after its pass)
-      fused = mode_ == FSIG_MODE_COUNT && p + 2 == k_ && c_->tune.fused_tail == 0 && tail_applies(pv_.level(p + 1));
+      fused = mode_ == FSIG_MODE_COUNT && p + 2 == k_ && c_->tune.fused_tail == 0 && tail_applies(pv_.level(p + 1)) &&
+              L.nb <= kTailMax && L.ncf <= kTailMax;
     if (last && mode_ == FSIG_MODE_COUNT) {
       FSIG_HIP(c_, hipMemsetAsync(res, 0, sizeof(unsigned long long), c_->stream));
       JoinIO io{rows, 0, R, nullptr, nullptr, nullptr, 0, 0, res, family_counters(c_, FSIG_K_JOIN_COUNT)};
@@ -819,11 +1018,19 @@ class JoinDriver {
                  const JoinLevel& L2, const CandEdgeSrc& S2) {
     unsigned long long* res = c_->d_counters;
     FSIG_HIP(c_, hipMemsetAsync(res, 0, 2 * sizeof(unsigned long long), c_->stream));
-    const uint32_t G = std::max<uint32_t>(G_, 8);
+    ChainTail chain;
+    const bool use_chain = chain_tail_applies(L1, L2) && pv_.chain_tail(L1.p, chain);
+    // lanes per parent row: at least 8 where the shared intersection goes
+    // through the LDS tile, else few (rows are mostly a handful of dependent
+    // gathers: more of them in flight)
+    const bool needs_tile = tail_tile_words(8, L2, 0) != 0;
+    const uint32_t G = use_chain || !needs_tile ? std::min<uint32_t>(std::max<uint32_t>(G_, 4), 8)
+                                                : std::max<uint32_t>(G_, 8);
     const TailIO io{rows, R, cnt, res, family_counters(c_, FSIG_K_JOIN_TAIL),
-                    tail_tile_words(G, L2, c_->tune.lds_tile)};
+                    use_chain ? 0u : tail_tile_words(G, L2, c_->tune.lds_tile)};
     KTimer t(c_, FSIG_K_JOIN_TAIL);
-    int rc = launch_join_tail(c_, G, L1, S1, L2, S2, io);
+    int rc = use_chain ? launch_join_tail_chain(c_, G, L1, S1, L2, S2, chain, io)
+                       : launch_join_tail(c_, G, L1, S1, L2, S2, io);
     if (rc) return rc;
     t.done(0, 0);
     uint64_t h[2];

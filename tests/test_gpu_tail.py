@@ -79,3 +79,31 @@ def test_tail_lockstep_batch(gpu_ctx):
                 gpu_ctx.set_tune(0, 0, 0, 4, 0)
             assert got.tolist() == want, (G, fused)
     dg.free()
+
+
+CHAINS = [
+    ([0, 1, 0, 1, 0, 1, 0, 1], [(i, i + 1) for i in range(7)]),         # same-label columns all along
+    ([0, 0, 0, 0, 0, 0], [(i, i + 1) for i in range(5)]),               # one label: every column conflicts
+    ([1, 0, 1, 1, 0, 1], [(0, 1), (1, 2), (1, 3), (3, 4), (4, 5)]),     # a tree ending in a chain
+    ([0, 1, 2, 0, 1], [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0), (0, 2)]),  # not a chain end (control)
+]
+
+
+@pytest.mark.parametrize("graph_labels", [2, 3])
+def test_chain_tail_equals_level_by_level_and_oracle(gpu_ctx, graph_labels):
+    """Queries whose last vertex hangs off the previous position, which has
+    one backward list (the chain tail): its count equals the level-by-level
+    count and the oracle, with conflicts of the same and of other classes."""
+    g = F.HostGraph.chung_lu(3000, 15000, 2.3, 70 + graph_labels).set_labels("uniform", graph_labels, seed=71)
+    off, adj, lab = g.arrays()
+    dg = gpu_ctx.upload(g)
+    for labels, edges in CHAINS:
+        q = F.Query(len(labels), [l % graph_labels for l in labels], edges)
+        want, _ = O.match(q.labels, q.adjm(), off, adj, lab, threads=8, limit=5 * 10 ** 8)
+        fused = {G: _count(gpu_ctx, dg, q, True, 0, G) for G in (4, 8, 16)}
+        plain = _count(gpu_ctx, dg, q, False)
+        print(labels, want, fused, plain, flush=True)
+        assert all(v == plain for v in fused.values())
+        if want != O.TOO_MANY:
+            assert plain == want
+    dg.free()
