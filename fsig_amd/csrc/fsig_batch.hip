@@ -1287,11 +1287,25 @@ struct StageClock {
 static std::map<std::pair<const fsig_ctx*, uint64_t>, TuneChoice> g_tuned;
 static std::mutex g_tuned_mu;
 
-// Level-p rows of one query beyond which a bounded query leaves the lockstep
-// join for the bounded depth-first driver (fsig_tune.spill_log2 overrides):
-// past this size the chunked driver, which stops at the bound, does less
-// work than a whole level.
-constexpr uint32_t kSpillLog2 = 20;
+// Beam of a bounded query in the lockstep join: the children of its first
+// parent rows up to this many (fsig_tune.beam_log2 overrides).  Queries with
+// more embeddings than their bound usually find them inside the beam, all in
+// the batch's shared launches.
+constexpr uint32_t kBeamLog2 = 16;
+
+// Rows of a lockstep level whose first child index reaches their query's cut
+// get no children (cnt = 0).
+__global__ void k_beam_cut(const uint64_t* __restrict__ off, const uint64_t* __restrict__ qbase, uint32_t nlive,
+                           const uint64_t* __restrict__ cut, uint64_t nrows, uint32_t* __restrict__ cnt) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = nlive;  // query of the row: last j with qbase[j] <= r
+    while (hi - lo > 1) {
+      const uint32_t m = (lo + hi) >> 1;
+      if (qbase[m] <= r) lo = m; else hi = m;
+    }
+    if (off[r] >= cut[lo]) cnt[r] = 0;
+  }
+}
 
 // Level descriptions of one query of a batch for the single-query driver.
 struct BatchProvider {
@@ -1337,6 +1351,7 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
   std::vector<uint8_t> settled(nq, 0);
   std::vector<uint64_t> settled_count(nq, 0);
   std::vector<int> settled_rc(nq, FSIG_OK);
+  std::vector<uint8_t> beamed(nq, 0);  // lockstep kept only part of its rows: its count is a lower bound
   const uint32_t n = g->n;
   Arena& A = c->arena;
   const size_t mark = A.mark();
@@ -1865,15 +1880,20 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
     }
     return pick_lanes(c, rows ? (double)edges / (double)rows : 1.0);
   };
-  // Bounded queries at a lockstep level, after its count pass and scan: a
-  // query whose count already reached its bound is settled at the bound; a
-  // query whose next level would exceed kSpillRows rows is finished from its
-  // level-p rows by the bounded depth-first driver, which stops at the bound.
-  // Either way its rows get no children here, and the level is rescanned.
-  const uint64_t spill_rows = 1ull << (c->tune.spill_log2 ? c->tune.spill_log2 : kSpillLog2);
+  // Bounded queries at a lockstep level, after its count pass and scan.  A
+  // query whose count already reached its bound is settled at the bound (its
+  // rows get no children).  A query whose next level would exceed the beam
+  // keeps only its first parent rows, those whose children start within the
+  // beam (the later rows get none): from then on its count is a lower bound
+  // of its count, which settles it once it reaches the bound; a beamed query
+  // that ends below its bound is counted again in full afterwards, by the
+  // bounded depth-first driver.  The level is then rescanned.
+  const uint64_t beam = 1ull << (c->tune.beam_log2 ? c->tune.beam_log2 : kBeamLog2);
   auto settle_bounded = [&](uint32_t p, const uint32_t* rows, uint32_t* cnt, uint64_t* off, uint64_t R,
                             const uint64_t* dqb, uint64_t* dqb2, const std::vector<uint64_t>& qb,
                             const std::vector<uint32_t>& live, uint64_t& T) -> int {
+    (void)p;
+    (void)rows;
     const uint32_t nl = (uint32_t)live.size();
     k_gather_bases<<<(nl + 256) / 256, 256, 0, c->stream>>>(off, dqb, nl, dqb2);
     FSIG_HIP(c, hipGetLastError());
@@ -1883,33 +1903,31 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
       if ((rc2 = read_u64(c, dqb2 + j, nb.data() + j, std::min<size_t>(kPinnedWords, nl + 1 - j)))) return rc2;
     for (uint32_t i = 0; i < nq; i += kPinnedWords)
       if ((rc2 = read_u64(c, dcounts + i, hc.data() + i, std::min<size_t>(kPinnedWords, nq - i)))) return rc2;
-    bool dropped = false;
+    // per live query: rows whose first child index is at or past cut[j] get
+    // no children
+    std::vector<uint64_t> cut(nl, ~0ull);
+    bool changed = false;
     for (uint32_t j = 0; j < nl; j++) {
       const uint32_t i = live[j];
       if (!bound[i] || settled[i] || qb[j + 1] == qb[j]) continue;
       if (hc[i] >= bound[i]) {
         settled[i] = 1;
         settled_count[i] = bound[i];
-      } else if (nb[j + 1] - nb[j] > spill_rows) {
-        BatchProvider pv;
-        pv.fill = [&fill_level, i](uint32_t pp, JoinLevel& L, CandEdgeSrc& S) { fill_level(i, pp, L, S); };
-        JoinDriver<BatchProvider> drv(c, pv, ks[i], lanes_for(i), FSIG_MODE_COUNT, bound[i], nullptr);
-        const size_t dm = A.mark();
-        rc2 = drv.run_at(p, rows + qb[j] * p, qb[j + 1] - qb[j]);
-        A.release(dm);
-        if (rc2 && rc2 != FSIG_ELIMIT) return rc2;
-        settled[i] = 1;
-        settled_count[i] = rc2 ? 0 : drv.total();
-        settled_rc[i] = rc2;  // over the context's work limit: refused
-      } else {
-        continue;
+        cut[j] = 0;
+        changed = true;
+      } else if (nb[j + 1] - nb[j] > beam) {
+        beamed[i] = 1;
+        cut[j] = nb[j] + beam;
+        changed = true;
       }
-      k_fill_u32<<<bgrid(c, qb[j + 1] - qb[j], FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(cnt + qb[j], qb[j + 1] - qb[j],
-                                                                                       0);
-      dropped = true;
     }
-    if (!dropped) return FSIG_OK;
+    if (!changed) return FSIG_OK;
+    const size_t mk2 = A.mark();
+    const uint64_t* dcut = upload_table(c, cut);
+    if (!dcut) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted");
+    k_beam_cut<<<bgrid(c, R, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(off, dqb, nl, dcut, R, cnt);
     FSIG_HIP(c, hipGetLastError());
+    A.release(mk2);  // the cut table is read before any later allocation is written
     if ((rc2 = scan_u32_to_u64(c, cnt, off, R))) return rc2;
     return read_u64(c, off + R, &T, 1);
   };
@@ -2166,6 +2184,34 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
   std::vector<uint64_t> hc(nq);
   for (uint32_t i = 0; i < nq; i += kPinnedWords)
     if ((rc = read_u64(c, dcounts + i, hc.data() + i, std::min<size_t>(kPinnedWords, nq - i)))) return rc;
+  // a beamed query whose beam found fewer embeddings than its bound is
+  // counted again in full by the bounded depth-first driver
+  clk.mark("lockstep join");
+  uint32_t nbeamed = 0, nagain = 0;
+  for (uint32_t i = 0; i < nq; i++) {
+    nbeamed += beamed[i];
+    nagain += beamed[i] && !settled[i] && hc[i] < bound[i];
+  }
+  if (clk.on) fprintf(stderr, "[fsig batch] beamed %u queries, %u counted again\n", nbeamed, nagain);
+  for (uint32_t i = 0; i < nq; i++) {
+    if (!beamed[i] || settled[i] || hc[i] >= bound[i]) continue;
+    const QueryPlan& P = plans[i];
+    const uint32_t n1 = size[cls_base[i] + P.cls[P.order[0]]];
+    const size_t dm = A.mark();
+    uint32_t* level1 = (uint32_t*)A.alloc(4ull * std::max<uint32_t>(n1, 1));
+    if (!level1) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted");
+    k_iota<<<bgrid(c, n1, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(level1, n1);
+    FSIG_HIP(c, hipGetLastError());
+    BatchProvider pv;
+    pv.fill = [&fill_level, i](uint32_t pp, JoinLevel& L, CandEdgeSrc& S) { fill_level(i, pp, L, S); };
+    JoinDriver<BatchProvider> drv(c, pv, ks[i], lanes_for(i), FSIG_MODE_COUNT, bound[i], nullptr);
+    const int rc2 = drv.run(level1, n1);
+    A.release(dm);
+    if (rc2 && rc2 != FSIG_ELIMIT) return rc2;
+    settled[i] = 1;
+    settled_count[i] = rc2 ? 0 : drv.total();
+    settled_rc[i] = rc2;  // over the context's work limit: refused
+  }
   for (uint32_t i = 0; i < nq; i++) {
     counts_out[i] = active[i] ? hc[i] : 0;
     if (active[i] && ks[i] == 1) counts_out[i] = size[cls_base[i] + plans[i].cls[0]];

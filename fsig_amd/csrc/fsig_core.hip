@@ -600,7 +600,7 @@ int fsig_set_tune(fsig_ctx* c, const fsig_tune* t) {
   if (t->strategy > 3) return set_err(c, FSIG_EINVAL, "tune: strategy");
   if (t->fused_tail > 1) return set_err(c, FSIG_EINVAL, "tune: fused_tail");
   if (t->nlf_filter > 1) return set_err(c, FSIG_EINVAL, "tune: nlf_filter");
-  if (t->spill_log2 > 40) return set_err(c, FSIG_EINVAL, "tune: spill_log2");
+  if (t->beam_log2 > 40) return set_err(c, FSIG_EINVAL, "tune: beam_log2");
   c->tune = *t;
   return FSIG_OK;
 }

@@ -171,10 +171,13 @@ int fsig_match_batch(fsig_ctx* ctx, const fsig_graph* g, uint32_t nq, const uint
  * query i exactly; otherwise counts_out[i] = min(count, max_embeddings[i])
  * (fsig_match's max_embeddings: pass L+1 to ask "more than L?") and its
  * search stops once that many embeddings are found.  A bounded query joins
- * the lockstep batch until its count reaches the bound or its next level
- * would exceed 2^20 rows (fsig_tune.spill_log2); it is then finished, from the rows it reached, by
- * fsig_match's memory-blocked depth-first driver, which stops at the bound.
- * That driver honours the context's work limit (fsig_set_work_limit): a
+ * the lockstep batch; when its next level would exceed the beam (2^16
+ * children, fsig_tune.beam_log2) only its first parent rows, whose children
+ * start within the beam, go on, and the count found from then on is a lower
+ * bound that settles the query once it reaches the bound.  A beamed query
+ * that ends below its bound is counted again in full by fsig_match's
+ * memory-blocked depth-first driver, which stops at the bound.  That driver
+ * honours the context's work limit (fsig_set_work_limit): a
  * query over it is refused, status_out[i] = FSIG_ELIMIT and counts_out[i]
  * = 0, while the other queries complete (status_out NULL: the call then
  * returns FSIG_ELIMIT).  No autotuning (lockstep with lanes per query).  No
@@ -240,9 +243,9 @@ typedef struct fsig_tune {
                              least as many neighbours of that label as the
                              query vertex has (graphs with 2..255 labels);
                              1: label and degree only, as the reference */
-  uint32_t spill_log2;    /* fsig_match_batch_bounded: a bounded query leaves
-                             the lockstep join when its next level would
-                             exceed 2^spill_log2 rows (0: 20; 1..40) */
+  uint32_t beam_log2;     /* fsig_match_batch_bounded: beam of a bounded query
+                             in the lockstep join, 2^beam_log2 children
+                             (0: 16; 1..40) */
 } fsig_tune;
 int fsig_set_tune(fsig_ctx* ctx, const fsig_tune* t);
 int fsig_get_tune(const fsig_ctx* ctx, fsig_tune* t);

@@ -119,13 +119,14 @@ def test_batch_hub_candidates(gpu_ctx):
     dg.free()
 
 
-@pytest.mark.parametrize("spill_log2", [0, 4])
-def test_bounded_batch_equals_bounded_plain_and_oracle(gpu_ctx, spill_log2):
+@pytest.mark.parametrize("beam_log2", [0, 4])
+def test_bounded_batch_equals_bounded_plain_and_oracle(gpu_ctx, beam_log2):
     """fsig_match_batch_bounded: min(count, bound) per query, equal to
     fsig_match's bounded count and to the oracle (stopped searches prove
-    the count exceeds the bound).  spill_log2 = 4 sends every bounded query
-    with a level above 16 rows to the depth-first driver; exact queries
-    (bound 0) stay in the lockstep join."""
+    the count exceeds the bound).  beam_log2 = 4 keeps a 16-child beam of
+    every bounded query in the lockstep join, so most of them end below their
+    bound and are counted again by the depth-first driver; exact queries
+    (bound 0) are never beamed."""
     g = F.HostGraph.chung_lu(20000, 120000, 2.3, 97).set_labels("zipf", 6, 1.2, 98)
     off, adj, lab = g.arrays()
     dg = gpu_ctx.upload(g)
@@ -136,7 +137,7 @@ def test_bounded_batch_equals_bounded_plain_and_oracle(gpu_ctx, spill_log2):
         if bounds[i] == 0 and F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT, max_embeddings=10 ** 7 + 1).count > 10 ** 7:
             bounds[i] = 10 ** 6
     try:
-        gpu_ctx.set_tune(spill_log2=spill_log2)
+        gpu_ctx.set_tune(beam_log2=beam_log2)
         counts, _, st = F.match_batch(dg, qs, bounds=bounds, status=True)
     finally:
         gpu_ctx.set_tune()
@@ -160,7 +161,7 @@ def test_bounded_batch_work_limit_refuses_per_query(gpu_ctx):
     light = F.Query(2, [0, 1], [(0, 1)])
     want_light = F.match(dg, light, F.ALG_MMATCH, F.MODE_COUNT).count
     try:
-        gpu_ctx.set_tune(spill_log2=4)
+        gpu_ctx.set_tune(beam_log2=4)
         gpu_ctx.set_work_limit(1000)
         counts, _, st = F.match_batch(dg, [heavy, light], bounds=[10 ** 12, 10 ** 12], status=True)
     finally:
