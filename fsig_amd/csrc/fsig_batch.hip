@@ -35,6 +35,7 @@
 #include "device.h"
 #include "internal.h"
 #include "join.h"
+#include "cedge.h"
 
 namespace {
 
@@ -224,6 +225,7 @@ struct BPair {
   const uint32_t* acand;
   const uint32_t* bcand;
   const uint32_t* mask;
+  const uint32_t* bits;  // candidate bitmap of b (null: the mask)
   uint32_t* off_out;
   const uint32_t* bpos;  // vertex -> candidate index of b (null: binary search)
   uint32_t asize, bsize, bbit, pad;
@@ -253,10 +255,10 @@ struct BSrc {
   const uint32_t* bits[MAXQ];  // candidate bitmap (n bits) of target class b
 };
 
-// Target classes among tm that vertex w is a candidate of: one gather of
-// the query's class-mask word, or (graphs of at most 4M vertices) one bit
-// test per target in n-bit class bitmaps, which stay in L2 there while the
-// 4-byte-per-vertex masks of a whole batch do not.
+// Target classes among tm that vertex w is a candidate of: one bit test per
+// target in n-bit class bitmaps, which stay in L2 while the 4-byte-per-vertex
+// masks of a whole batch do not, or (no room for bitmaps) one gather of the
+// query's class-mask word.
 __device__ __forceinline__ uint32_t target_bits(const BSrc& sc, uint32_t w, uint32_t tm) {
   if (sc.mask) return sc.mask[w] & tm;
   uint32_t m = 0;
@@ -286,9 +288,8 @@ __global__ void k_bset_bits(const BBits* __restrict__ T, uint32_t nt, uint64_t t
   }
 }
 
-// Candidates of more than kHubDeg neighbours are swept by k_bcedge_hub (a
-// workgroup per candidate) instead of one lane group.
-constexpr uint32_t kHubDeg = 2048;
+// Candidates of more than kHubDeg neighbours are cut into segments
+// (k_seg_cedge, per pair) instead of one lane group.
 
 template <int G>
 __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_count_multi(const BSrc* __restrict__ S, uint32_t ns,
@@ -307,7 +308,7 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_count_multi(const BSrc* _
     const uint32_t v = sc.acand[i0];
     const uint32_t tm = sc.tmask;
     const uint32_t s = off[v], e = off[v + 1];
-    if (e - s > kHubDeg) continue;  // k_bcedge_hub
+    if (e - s > kHubDeg) continue;  // k_seg_cedge
     if (gl == 0)
       for (uint32_t t = tm; t; t &= t - 1) acc[gw][__builtin_ctz(t)] = 0;
     for (uint32_t j = s; j < e; j += G) {
@@ -346,7 +347,7 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_fill_multi(const BSrc* __
     const uint32_t v = sc.acand[i0];
     const uint32_t tm = sc.tmask;
     const uint32_t s = off[v], e = off[v + 1];
-    if (e - s > kHubDeg) continue;  // k_bcedge_hub
+    if (e - s > kHubDeg) continue;  // k_seg_cedge
     if (gl == 0)
       for (uint32_t t = tm; t; t &= t - 1) {
         const uint32_t b = __builtin_ctz(t);
@@ -386,113 +387,6 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_fill_multi(const BSrc* __
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
     }
-  }
-}
-
-// Hub candidates (degree > kHubDeg) of all sources, gathered into one list
-// so that k_bcedge_hub deals them out one per workgroup (hubs cluster at the
-// low vertex ids of power-law graphs, i.e. at the start of every class).
-__global__ void k_bfind_hubs(const BSrc* __restrict__ S, uint32_t ns, uint64_t total,
-                             const uint32_t* __restrict__ off, uint64_t* __restrict__ hubs,
-                             unsigned int* __restrict__ nhubs) {
-  for (uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; f < total;
-       f += (uint64_t)gridDim.x * blockDim.x) {
-    const BSrc& sc = S[find_flat(S, ns, f)];
-    const uint32_t v = sc.acand[f - sc.flat];
-    if (off[v + 1] - off[v] > kHubDeg) hubs[atomicAdd(nhubs, 1u)] = f;
-  }
-}
-
-// Candidate edges of one hub per workgroup: its 4 wavefronts sweep one
-// contiguous quarter of the adjacency each, so a hub costs deg/256 steps
-// instead of deg/G on one lane group.  FILL == false: counts per target
-// class; FILL == true: each wavefront writes its quarter's entries after the
-// earlier quarters', so lists stay ascending.
-template <bool FILL>
-__global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_hub(const BSrc* __restrict__ S, uint32_t ns,
-                                                           const uint64_t* __restrict__ hubs,
-                                                           const unsigned int* __restrict__ nhubs,
-                                                           const BPair* __restrict__ P,
-                                                           const uint32_t* __restrict__ off,
-                                                           const uint32_t* __restrict__ adj,
-                                                           uint32_t* __restrict__ cnt,
-                                                           const uint64_t* __restrict__ gscan,
-                                                           uint32_t* __restrict__ lst_all) {
-  constexpr uint32_t W = FSIG_BLOCK / 64;
-  __shared__ uint32_t wcnt[W][MAXQ];
-  __shared__ uint64_t wat[W][MAXQ];
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const uint64_t below = (1ull << lane) - 1;
-  const uint32_t n_h = *nhubs;
-  for (uint32_t h = blockIdx.x; h < n_h; h += gridDim.x) {
-    const uint64_t fh = hubs[h];
-    const BSrc& sc = S[find_flat(S, ns, fh)];
-    const uint32_t i0 = (uint32_t)(fh - sc.flat);
-    const uint32_t v = sc.acand[i0];
-    const uint32_t tm = sc.tmask;
-    const uint32_t s = off[v], len = off[v + 1] - s, q = (len + W - 1) / W;
-    const uint32_t ws = s + min(len, wave * q), we = s + min(len, (wave + 1) * q);
-    if (lane == 0)
-      for (uint32_t t = tm; t; t &= t - 1) wcnt[wave][__builtin_ctz(t)] = 0;
-    lds_wave_sync();
-    for (uint32_t j = ws; j < we; j += 64) {
-      const uint32_t i = j + lane;
-      const uint32_t m = i < we ? target_bits(sc, adj[i], tm) : 0u;
-      for (uint32_t t = tm; t; t &= t - 1) {
-        const uint32_t b = __builtin_ctz(t);
-        const uint32_t got = (uint32_t)__popcll(__ballot((m >> b) & 1u));
-        if (lane == 0) wcnt[wave][b] += got;
-      }
-    }
-    __syncthreads();
-    if (!FILL) {
-      if (threadIdx.x == 0)
-        for (uint32_t t = tm; t; t &= t - 1) {
-          const uint32_t b = __builtin_ctz(t);
-          uint32_t sum = 0;
-          for (uint32_t w = 0; w < W; w++) sum += wcnt[w][b];
-          cnt[P[sc.pidx[b]].flat + i0] = sum;
-        }
-    } else {
-      if (lane == 0)
-        for (uint32_t t = tm; t; t &= t - 1) {
-          const uint32_t b = __builtin_ctz(t);
-          const BPair& pd = P[sc.pidx[b]];
-          const uint64_t a0 = gscan[pd.flat + i0];
-          uint64_t pre = 0;
-          for (uint32_t w = 0; w < wave; w++) pre += wcnt[w][b];
-          wat[wave][b] = a0 + pre;
-          if (wave == 0) {
-            const uint64_t pbase = gscan[pd.flat];
-            pd.off_out[i0] = (uint32_t)(a0 - pbase);
-            if (i0 + 1 == pd.asize) pd.off_out[pd.asize] = (uint32_t)(gscan[pd.flat + i0 + 1] - pbase);
-          }
-        }
-      lds_wave_sync();
-      for (uint32_t j = ws; j < we; j += 64) {
-        const uint32_t i = j + lane;
-        uint32_t x = 0, m = 0;
-        if (i < we) {
-          x = adj[i];
-          m = target_bits(sc, x, tm);
-        }
-        for (uint32_t t = tm; t; t &= t - 1) {
-          const uint32_t b = __builtin_ctz(t);
-          const bool hit = (m >> b) & 1u;
-          const uint64_t bal = __ballot(hit);
-          if (bal == 0) continue;
-          const uint64_t a = wat[wave][b];
-          if (hit) {
-            const BPair& pd = P[sc.pidx[b]];
-            lst_all[a + __popcll(bal & below)] = pd.bpos ? pd.bpos[x] : sorted_index(pd.bcand, pd.bsize, x);
-          }
-          lds_wave_sync();
-          if (lane == 0) wat[wave][b] = a + __popcll(bal);
-          lds_wave_sync();
-        }
-      }
-    }
-    __syncthreads();
   }
 }
 
@@ -1294,16 +1188,22 @@ static std::mutex g_tuned_mu;
 constexpr uint32_t kBeamLog2 = 16;
 
 // Rows of a lockstep level whose first child index reaches their query's cut
-// get no children (cnt = 0).
+// get no children (cnt = 0); first[j] receives query j's first such row (a
+// query's rows and their child offsets ascend, so its cut rows are a suffix
+// of its range; first[j] keeps its initial value when nothing is cut).
 __global__ void k_beam_cut(const uint64_t* __restrict__ off, const uint64_t* __restrict__ qbase, uint32_t nlive,
-                           const uint64_t* __restrict__ cut, uint64_t nrows, uint32_t* __restrict__ cnt) {
+                           const uint64_t* __restrict__ cut, uint64_t nrows, uint32_t* __restrict__ cnt,
+                           uint64_t* __restrict__ first) {
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t lo = 0, hi = nlive;  // query of the row: last j with qbase[j] <= r
     while (hi - lo > 1) {
       const uint32_t m = (lo + hi) >> 1;
       if (qbase[m] <= r) lo = m; else hi = m;
     }
-    if (off[r] >= cut[lo]) cnt[r] = 0;
+    if (off[r] >= cut[lo]) {
+      cnt[r] = 0;
+      if (r == qbase[lo] || off[r - 1] < cut[lo]) first[lo] = r;
+    }
   }
 }
 
@@ -1584,7 +1484,7 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
         if (pair_of[i][a * MAXQ + b] >= 0) continue;
         pair_of[i][a * MAXQ + b] = (int)pairs.size();
         const uint32_t ga = cls_base[i] + a, gb = cls_base[i] + b;
-        BPair pd{cand[ga], cand[gb], masks + (uint64_t)i * n, nullptr, nullptr, size[ga], size[gb], b, 0, pflat};
+        BPair pd{cand[ga], cand[gb], masks + (uint64_t)i * n, nullptr, nullptr, nullptr, size[ga], size[gb], b, 0, pflat};
         pd.off_out = (uint32_t*)A.alloc(4ull * (size[ga] + 1));
         if (!pd.off_out) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted for candidate edges");
         pflat += size[ga];
@@ -1616,22 +1516,27 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
         pd.bpos = it->second;
       }
     }
-    // candidate bitmaps of every target class, n bits each (graphs of at
-    // most 4M vertices)
+    // candidate bitmaps of every target class, n bits each, while they take
+    // at most a quarter of the arena's free space: a bitmap stays in L2
+    // while one pair's entries are tested, a query's 4n-byte class mask
+    // does not on large graphs
     const uint64_t nwords = (n + 31ull) / 32;
-    const bool use_bits = n <= (1u << 22);
     std::vector<uint32_t*> tbits(std::max<uint32_t>(ncls, 1), nullptr);
-    if (use_bits) {
-      std::vector<BBits> bt;
-      uint64_t bflat = 0;
-      std::vector<uint32_t> tgt;
+    std::vector<uint32_t> tgt;
+    {
+      std::vector<uint8_t> seen(std::max<uint32_t>(ncls, 1), 0);
       for (uint32_t i = 0; i < nq; i++)
         for (uint32_t x = 0; x < MAXQ * MAXQ && active[i]; x++)
           if (pair_of[i][x] >= 0) {
             const uint32_t gb = cls_base[i] + x % MAXQ;
-            if (!tbits[gb]) { tbits[gb] = reinterpret_cast<uint32_t*>(1); tgt.push_back(gb); }
+            if (!seen[gb]) { seen[gb] = 1; tgt.push_back(gb); }
           }
-      uint32_t* base = (uint32_t*)A.alloc(4ull * nwords * std::max<size_t>(tgt.size(), 1));
+    }
+    const bool use_bits = !tgt.empty() && 4ull * nwords * tgt.size() <= A.avail() / 4;
+    if (use_bits) {
+      std::vector<BBits> bt;
+      uint64_t bflat = 0;
+      uint32_t* base = (uint32_t*)A.alloc(4ull * nwords * tgt.size());
       if (!base) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted for candidate bitmaps");
       for (size_t j = 0; j < tgt.size(); j++) {
         tbits[tgt[j]] = base + j * nwords;
@@ -1640,13 +1545,15 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
           bflat += size[tgt[j]];
         }
       }
-      if (!tgt.empty())
-        k_fill_u32<<<bgrid(c, nwords * tgt.size(), FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(base, nwords * tgt.size(), 0);
+      k_fill_u32<<<bgrid(c, nwords * tgt.size(), FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(base, nwords * tgt.size(), 0);
       const BBits* dbt = upload_table(c, bt);
       if (!dbt) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted");
       if (bflat)
         k_bset_bits<<<bgrid(c, bflat, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(dbt, (uint32_t)bt.size(), bflat);
       FSIG_HIP(c, hipGetLastError());
+      for (uint32_t i = 0; i < nq; i++)
+        for (uint32_t x = 0; x < MAXQ * MAXQ && active[i]; x++)
+          if (pair_of[i][x] >= 0) pairs[pair_of[i][x]].bits = tbits[cls_base[i] + x % MAXQ];
     }
     // sources: every (query, class a) with its targets, one sweep each
     std::vector<BSrc> srcs;
@@ -1688,18 +1595,9 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
         default: k_bcedge_count_multi<64><<<bgrid(c, sflat, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, cnt); break;
       }
     }
-    const uint32_t hub_grid = (uint32_t)c->num_cus * 4;
-    uint64_t* hubs = nullptr;
-    unsigned int* nhubs = nullptr;
-    if (sflat && g->max_degree > kHubDeg) {
-      hubs = (uint64_t*)A.alloc(8ull * sflat);
-      nhubs = (unsigned int*)A.alloc(4);
-      if (!hubs || !nhubs) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted");
-      k_fill_u32<<<1, 64, 0, c->stream>>>(nhubs, 1, 0);
-      k_bfind_hubs<<<bgrid(c, sflat, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, g->d_off, hubs, nhubs);
-      k_bcedge_hub<false><<<hub_grid, FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, hubs, nhubs, dP, g->d_off, g->d_adj, cnt,
-                                                                  nullptr, nullptr);
-    }
+    SegRows segs;  // hub rows of every pair, one wavefront per segment
+    if ((rc = seg_prepare(c, dP, np, pflat, g->d_off, cnt, segs))) return rc;
+    if ((rc = seg_count(c, dP, np, pflat, g->d_off, g->d_adj, cnt, segs))) return rc;
     FSIG_HIP(c, hipGetLastError());
     if ((rc = scan_u32_to_u64(c, cnt, gscan, pflat))) return rc;
     uint64_t total_edges;
@@ -1713,9 +1611,7 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
         default: k_bcedge_fill_multi<64><<<bgrid(c, sflat, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, gscan, lst_all); break;
       }
     }
-    if (hubs)
-      k_bcedge_hub<true><<<hub_grid, FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, hubs, nhubs, dP, g->d_off, g->d_adj,
-                                                                 nullptr, gscan, lst_all);
+    if ((rc = seg_fill(c, dP, np, pflat, g->d_off, g->d_adj, gscan, lst_all, segs))) return rc;
     k_bpair_maxlen<<<np, FSIG_BLOCK, 0, c->stream>>>(dP, gscan, dmax);
     k_bpair_base<<<(np + 255) / 256, 256, 0, c->stream>>>(gscan, dP, np, pb);
     FSIG_HIP(c, hipGetLastError());
@@ -1726,7 +1622,9 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
     FSIG_HIP(c, hipStreamSynchronize(c->stream));
     for (uint32_t i = 0; i < np; i++) maxlen_h[i] = mx[i];
     for (uint32_t i = 0; i < np; i++) ptotal_h[i] = (i + 1 < np ? pbase_h[i + 1] : total_edges) - pbase_h[i];
-    tc.done(2ull * pflat * 12 + 12ull * total_edges, pflat);
+    // bytes as the per-query build's, with sources' rows read once for all
+    // their targets
+    tc.done(28ull * pflat + 16ull * segs.entries + 8ull * total_edges + 4ull * (pflat + np), segs.entries);
   }
 
   clk.mark("candidate edges");
@@ -1885,15 +1783,21 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
   // rows get no children).  A query whose next level would exceed the beam
   // keeps only its first parent rows, those whose children start within the
   // beam (the later rows get none): from then on its count is a lower bound
-  // of its count, which settles it once it reaches the bound; a beamed query
-  // that ends below its bound is counted again in full afterwards, by the
-  // bounded depth-first driver.  The level is then rescanned.
+  // of its count, which settles it once it reaches the bound.  The cut rows
+  // (a suffix of the query's rows at that level) are recorded; a beamed
+  // query that ends below its bound has the subtrees of its cut rows counted
+  // afterwards by the bounded depth-first driver, from those rows, while the
+  // level tables are still held.  The level is then rescanned.
   const uint64_t beam = 1ull << (c->tune.beam_log2 ? c->tune.beam_log2 : kBeamLog2);
+  struct CutRows {
+    uint32_t p;            // level of the rows (p words each)
+    const uint32_t* rows;  // in that level's table
+    uint64_t n;
+  };
+  std::vector<std::vector<CutRows>> cut_rows(nq);
   auto settle_bounded = [&](uint32_t p, const uint32_t* rows, uint32_t* cnt, uint64_t* off, uint64_t R,
                             const uint64_t* dqb, uint64_t* dqb2, const std::vector<uint64_t>& qb,
                             const std::vector<uint32_t>& live, uint64_t& T) -> int {
-    (void)p;
-    (void)rows;
     const uint32_t nl = (uint32_t)live.size();
     k_gather_bases<<<(nl + 256) / 256, 256, 0, c->stream>>>(off, dqb, nl, dqb2);
     FSIG_HIP(c, hipGetLastError());
@@ -1923,13 +1827,45 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
     }
     if (!changed) return FSIG_OK;
     const size_t mk2 = A.mark();
+    std::vector<uint64_t> first(qb.begin() + 1, qb.end());  // nothing cut: the query's end
     const uint64_t* dcut = upload_table(c, cut);
-    if (!dcut) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted");
-    k_beam_cut<<<bgrid(c, R, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(off, dqb, nl, dcut, R, cnt);
+    uint64_t* dfirst = (uint64_t*)upload_table(c, first);
+    if (!dcut || !dfirst) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted");
+    k_beam_cut<<<bgrid(c, R, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(off, dqb, nl, dcut, R, cnt, dfirst);
     FSIG_HIP(c, hipGetLastError());
-    A.release(mk2);  // the cut table is read before any later allocation is written
+    for (uint32_t j = 0; j < nl; j += kPinnedWords)
+      if ((rc2 = read_u64(c, dfirst + j, first.data() + j, std::min<size_t>(kPinnedWords, nl - j)))) return rc2;
+    A.release(mk2);  // both tables are read before any later allocation is written
+    for (uint32_t j = 0; j < nl; j++)
+      if (cut[j] != ~0ull && cut[j] != 0 && first[j] < qb[j + 1])
+        cut_rows[live[j]].push_back(CutRows{p, rows + first[j] * p, qb[j + 1] - first[j]});
     if ((rc2 = scan_u32_to_u64(c, cnt, off, R))) return rc2;
     return read_u64(c, off + R, &T, 1);
+  };
+  // The subtrees of the cut rows of every beamed query that ended below its
+  // bound, counted by the bounded depth-first driver (bound: what the beam
+  // left); settles those queries.
+  auto resume_beamed = [&]() -> int {
+    std::vector<uint64_t> hc(nq);
+    int rc2;
+    for (uint32_t i = 0; i < nq; i += kPinnedWords)
+      if ((rc2 = read_u64(c, dcounts + i, hc.data() + i, std::min<size_t>(kPinnedWords, nq - i)))) return rc2;
+    for (uint32_t i = 0; i < nq; i++) {
+      if (!beamed[i] || settled[i] || hc[i] >= bound[i]) continue;
+      BatchProvider pv;
+      pv.fill = [&fill_level, i](uint32_t pp, JoinLevel& L, CandEdgeSrc& S) { fill_level(i, pp, L, S); };
+      JoinDriver<BatchProvider> drv(c, pv, ks[i], lanes_for(i), FSIG_MODE_COUNT, bound[i] - hc[i], nullptr);
+      rc2 = FSIG_OK;
+      for (const CutRows& cr : cut_rows[i]) {
+        if (drv.total() >= bound[i] - hc[i]) break;
+        if ((rc2 = drv.run_at(cr.p, cr.rows, cr.n))) break;
+      }
+      if (rc2 && rc2 != FSIG_ELIMIT) return rc2;
+      settled[i] = 1;
+      settled_count[i] = rc2 ? 0 : hc[i] + drv.total();
+      settled_rc[i] = rc2;  // over the context's work limit: refused
+    }
+    return FSIG_OK;
   };
   // Lockstep batched BFS.  G = 0: lanes by role at each level: 64 for the
   // queries whose last two levels are counted here (long shared
@@ -2054,8 +1990,9 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
     FSIG_HIP(c, hipEventRecord(c->ev_b, c->stream));
     FSIG_HIP(c, hipEventSynchronize(c->ev_b));
     FSIG_HIP(c, hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
+    const int rc2 = any_bound ? resume_beamed() : FSIG_OK;
     A.release(mk);
-    return FSIG_OK;
+    return rc2;
   };
   // Persistent work-queue walk over the seed tasks of a set of queries.
   auto groups_for = [&](uint32_t G, uint32_t grid) { return (uint64_t)grid * (FSIG_BLOCK / G); };
@@ -2184,33 +2121,11 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
   std::vector<uint64_t> hc(nq);
   for (uint32_t i = 0; i < nq; i += kPinnedWords)
     if ((rc = read_u64(c, dcounts + i, hc.data() + i, std::min<size_t>(kPinnedWords, nq - i)))) return rc;
-  // a beamed query whose beam found fewer embeddings than its bound is
-  // counted again in full by the bounded depth-first driver
   clk.mark("lockstep join");
-  uint32_t nbeamed = 0, nagain = 0;
-  for (uint32_t i = 0; i < nq; i++) {
-    nbeamed += beamed[i];
-    nagain += beamed[i] && !settled[i] && hc[i] < bound[i];
-  }
-  if (clk.on) fprintf(stderr, "[fsig batch] beamed %u queries, %u counted again\n", nbeamed, nagain);
-  for (uint32_t i = 0; i < nq; i++) {
-    if (!beamed[i] || settled[i] || hc[i] >= bound[i]) continue;
-    const QueryPlan& P = plans[i];
-    const uint32_t n1 = size[cls_base[i] + P.cls[P.order[0]]];
-    const size_t dm = A.mark();
-    uint32_t* level1 = (uint32_t*)A.alloc(4ull * std::max<uint32_t>(n1, 1));
-    if (!level1) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted");
-    k_iota<<<bgrid(c, n1, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(level1, n1);
-    FSIG_HIP(c, hipGetLastError());
-    BatchProvider pv;
-    pv.fill = [&fill_level, i](uint32_t pp, JoinLevel& L, CandEdgeSrc& S) { fill_level(i, pp, L, S); };
-    JoinDriver<BatchProvider> drv(c, pv, ks[i], lanes_for(i), FSIG_MODE_COUNT, bound[i], nullptr);
-    const int rc2 = drv.run(level1, n1);
-    A.release(dm);
-    if (rc2 && rc2 != FSIG_ELIMIT) return rc2;
-    settled[i] = 1;
-    settled_count[i] = rc2 ? 0 : drv.total();
-    settled_rc[i] = rc2;  // over the context's work limit: refused
+  if (clk.on) {
+    uint32_t nbeamed = 0;
+    for (uint32_t i = 0; i < nq; i++) nbeamed += beamed[i];
+    fprintf(stderr, "[fsig batch] beamed %u queries\n", nbeamed);
   }
   for (uint32_t i = 0; i < nq; i++) {
     counts_out[i] = active[i] ? hc[i] : 0;

@@ -17,6 +17,7 @@
 #include "internal.h"
 #include "join.h"
 #include "common.h"
+#include "cedge.h"
 
 namespace {
 
@@ -145,173 +146,23 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_refine_apply(RefineTable T, cons
 struct PairDesc {
   const uint32_t* acand;
   const uint32_t* bcand;
+  const uint32_t* mask;  // the query's class mask
+  const uint32_t* bits;  // candidate bitmap of b (null: the mask)
   uint32_t* off_out;
   const uint32_t* bpos;  // vertex -> candidate index of b (null: binary search)
-  uint32_t asize, bsize, bbit, flat_base;
+  uint32_t asize, bsize, bbit, pad;
+  uint64_t flat;  // first row of the pair in the flat row space
 };
-
-// Candidates of more than kHubDeg neighbours are swept by k_cedge_hub (a
-// workgroup each) instead of one lane group.
-constexpr uint32_t kHubDeg = 2048;
 
 // bpos[C[j]] = j: the vertex -> candidate index map of one class.
 __global__ void k_scatter_pos(const uint32_t* __restrict__ cand, uint32_t n, uint32_t* __restrict__ pos) {
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) pos[cand[j]] = j;
 }
 
-__device__ __forceinline__ uint32_t find_pair(const PairDesc* P, uint32_t np, uint64_t f) {
-  uint32_t lo = 0, hi = np;  // last pair with flat_base <= f
-  while (hi - lo > 1) {
-    uint32_t m = (lo + hi) >> 1;
-    if (P[m].flat_base <= f) lo = m; else hi = m;
-  }
-  return lo;
-}
-
-template <int G>
-__global__ void __launch_bounds__(FSIG_BLOCK) k_cedge_count(const PairDesc* __restrict__ P, uint32_t np,
-                                                            uint32_t total, const uint32_t* __restrict__ off,
-                                                            const uint32_t* __restrict__ adj,
-                                                            const uint32_t* __restrict__ mask,
-                                                            uint32_t* __restrict__ cnt) {
-  constexpr int GPB = FSIG_BLOCK / G;
-  const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
-  const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
-  const uint64_t ngroups = (uint64_t)gridDim.x * GPB;
-  for (uint64_t f = (uint64_t)xcd_block() * GPB + gw; f < total; f += ngroups) {
-    const PairDesc& pd = P[find_pair(P, np, f)];
-    const uint32_t v = pd.acand[f - pd.flat_base];
-    const uint32_t s = off[v], e = off[v + 1];
-    if (e - s > kHubDeg) continue;  // k_cedge_hub
-    uint32_t got = 0;
-    for (uint32_t j = s; j < e; j += G) {
-      const uint32_t i = j + gl;
-      const bool hit = i < e && ((mask[adj[i]] >> pd.bbit) & 1u);
-      got += __popcll(__ballot(hit) & gmask);
-    }
-    if (gl == 0) cnt[f] = got;
-  }
-}
-
-template <int G>
-__global__ void __launch_bounds__(FSIG_BLOCK) k_cedge_fill(const PairDesc* __restrict__ P, uint32_t np,
-                                                           uint32_t total, const uint32_t* __restrict__ off,
-                                                           const uint32_t* __restrict__ adj,
-                                                           const uint32_t* __restrict__ mask,
-                                                           const uint64_t* __restrict__ gscan,
-                                                           uint32_t* __restrict__ lst_all) {
-  constexpr int GPB = FSIG_BLOCK / G;
-  const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
-  const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
-  const uint64_t below = (1ull << lane) - 1;
-  const uint64_t ngroups = (uint64_t)gridDim.x * GPB;
-  for (uint64_t f = (uint64_t)xcd_block() * GPB + gw; f < total; f += ngroups) {
-    const PairDesc& pd = P[find_pair(P, np, f)];
-    const uint32_t i0 = (uint32_t)(f - pd.flat_base);
-    const uint32_t v = pd.acand[i0];
-    const uint32_t s = off[v], e = off[v + 1];
-    if (e - s > kHubDeg) continue;  // k_cedge_hub
-    const uint64_t pbase = gscan[pd.flat_base];
-    uint64_t at = gscan[f];
-    if (gl == 0) {
-      pd.off_out[i0] = (uint32_t)(at - pbase);
-      if (i0 + 1 == pd.asize) pd.off_out[pd.asize] = (uint32_t)(gscan[f + 1] - pbase);
-    }
-    for (uint32_t j = s; j < e; j += G) {
-      const uint32_t i = j + gl;
-      uint32_t x = 0;
-      bool hit = false;
-      if (i < e) {
-        x = adj[i];
-        hit = (mask[x] >> pd.bbit) & 1u;
-      }
-      const uint64_t bal = __ballot(hit) & gmask;
-      if (hit) lst_all[at + __popcll(bal & below)] = pd.bpos ? pd.bpos[x] : sorted_index(pd.bcand, pd.bsize, x);
-      at += __popcll(bal);
-    }
-  }
-}
-
-// Hub candidates (degree > kHubDeg) of all pairs, gathered into one list so
-// that k_cedge_hub can deal them out one per workgroup (hubs cluster at the
-// low vertex ids of power-law graphs, i.e. at the start of every class).
-__global__ void k_find_hubs(const PairDesc* __restrict__ P, uint32_t np, uint32_t total,
-                            const uint32_t* __restrict__ off, uint32_t* __restrict__ hubs,
-                            unsigned int* __restrict__ nhubs) {
-  for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < total; f += gridDim.x * blockDim.x) {
-    const PairDesc& pd = P[find_pair(P, np, f)];
-    const uint32_t v = pd.acand[f - pd.flat_base];
-    if (off[v + 1] - off[v] > kHubDeg) hubs[atomicAdd(nhubs, 1u)] = f;
-  }
-}
-
-// Candidate edges of one hub per workgroup: the 4 wavefronts sweep one
-// contiguous quarter of its adjacency each (the fill writes each quarter
-// after the earlier ones, so lists stay ascending).
-template <bool FILL>
-__global__ void __launch_bounds__(FSIG_BLOCK) k_cedge_hub(const PairDesc* __restrict__ P, uint32_t np,
-                                                          const uint32_t* __restrict__ hubs,
-                                                          const unsigned int* __restrict__ nhubs,
-                                                          const uint32_t* __restrict__ off,
-                                                          const uint32_t* __restrict__ adj,
-                                                          const uint32_t* __restrict__ mask,
-                                                          uint32_t* __restrict__ cnt,
-                                                          const uint64_t* __restrict__ gscan,
-                                                          uint32_t* __restrict__ lst_all) {
-  constexpr uint32_t W = FSIG_BLOCK / 64;
-  __shared__ uint32_t wcnt[W];
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const uint64_t below = (1ull << lane) - 1;
-  const uint32_t n_h = *nhubs;
-  for (uint32_t h = blockIdx.x; h < n_h; h += gridDim.x) {
-    const uint32_t fh = hubs[h];
-    const PairDesc& pd = P[find_pair(P, np, fh)];
-    const uint32_t i0 = fh - pd.flat_base;
-    const uint32_t v = pd.acand[i0];
-    const uint32_t s = off[v], len = off[v + 1] - s, q = (len + W - 1) / W;
-    const uint32_t ws = s + min(len, wave * q), we = s + min(len, (wave + 1) * q);
-    uint32_t got = 0;
-    for (uint32_t j = ws; j < we; j += 64) {
-      const uint32_t i = j + lane;
-      got += (uint32_t)__popcll(__ballot(i < we && ((mask[adj[i]] >> pd.bbit) & 1u)));
-    }
-    if (lane == 0) wcnt[wave] = got;
-    __syncthreads();
-    if (!FILL) {
-      if (threadIdx.x == 0) {
-        uint32_t sum = 0;
-        for (uint32_t w = 0; w < W; w++) sum += wcnt[w];
-        cnt[fh] = sum;
-      }
-    } else {
-      const uint64_t pbase = gscan[pd.flat_base];
-      uint64_t at = gscan[fh];
-      if (threadIdx.x == 0) {
-        pd.off_out[i0] = (uint32_t)(at - pbase);
-        if (i0 + 1 == pd.asize) pd.off_out[pd.asize] = (uint32_t)(gscan[fh + 1] - pbase);
-      }
-      for (uint32_t w = 0; w < wave; w++) at += wcnt[w];
-      for (uint32_t j = ws; j < we; j += 64) {
-        const uint32_t i = j + lane;
-        uint32_t x = 0;
-        bool hit = false;
-        if (i < we) {
-          x = adj[i];
-          hit = (mask[x] >> pd.bbit) & 1u;
-        }
-        const uint64_t bal = __ballot(hit);
-        if (hit) lst_all[at + __popcll(bal & below)] = pd.bpos ? pd.bpos[x] : sorted_index(pd.bcand, pd.bsize, x);
-        at += __popcll(bal);
-      }
-    }
-    __syncthreads();
-  }
-}
-
 __global__ void k_gather_u64(const uint64_t* __restrict__ src, const PairDesc* __restrict__ P, uint32_t np,
                              uint64_t* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < np) out[i] = src[P[i].flat_base];
+  if (i < np) out[i] = src[P[i].flat];
 }
 
 }  // namespace
@@ -553,8 +404,11 @@ int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_
       pd.asize = size[a];
       pd.bsize = size[b];
       pd.bbit = b;
+      pd.mask = mask;
+      pd.bits = nullptr;
       pd.off_out = nullptr;
-      pd.flat_base = 0;
+      pd.pad = 0;
+      pd.flat = 0;
       pairs.push_back(pd);
     }
   }
@@ -577,8 +431,11 @@ int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_
     pd.asize = size[cls_w];
     pd.bsize = size[cls_p];
     pd.bbit = cls_p;
+    pd.mask = mask;
+    pd.bits = nullptr;
     pd.off_out = nullptr;
-    pd.flat_base = 0;
+    pd.pad = 0;
+    pd.flat = 0;
     pairs.push_back(pd);
   }
   double avg_list = 1;
@@ -587,7 +444,7 @@ int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_
     const uint32_t np = (uint32_t)pairs.size();
     uint64_t flat = 0;
     for (auto& pd : pairs) {
-      pd.flat_base = (uint32_t)flat;
+      pd.flat = flat;
       flat += pd.asize;
       pd.off_out = (uint32_t*)A.alloc(4ull * (pd.asize + 1));
       if (!pd.off_out) return set_err(c, FSIG_ENOMEM, "cedge: arena exhausted");
@@ -611,44 +468,40 @@ int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_
         for (auto& pd : pairs) pd.bpos = nullptr;
       for (uint32_t b = 0; b < nc && fit; b++) pv.pos[b] = bpos_of[b];  // the join's conflict lookups
     }
+    // candidate bitmaps of the target classes on graphs whose class mask
+    // outgrows L2 (n/8 bytes each instead of the mask's 4n)
+    if (g->n >= (1u << 20)) {
+      const uint64_t words = (g->n + 31ull) / 32;
+      uint32_t* bits_of[MAXQ] = {nullptr};
+      for (auto& pd : pairs) {
+        const uint32_t b = pd.bbit;
+        if (!bits_of[b]) {
+          bits_of[b] = (uint32_t*)A.alloc(4ull * words);
+          if (!bits_of[b]) return set_err(c, FSIG_ENOMEM, "cedge: arena exhausted");
+          k_fill_u32<<<grid_for(c, words, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(bits_of[b], words, 0);
+          k_set_bits<<<grid_for(c, pd.bsize, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(pd.bcand, pd.bsize, bits_of[b]);
+        }
+        pd.bits = bits_of[b];
+      }
+    }
     PairDesc* dP = (PairDesc*)A.alloc(sizeof(PairDesc) * np);
     uint32_t* cnt = (uint32_t*)A.alloc(4ull * flat);
     uint64_t* gscan = (uint64_t*)A.alloc(8ull * (flat + 1));
     uint64_t* pbase = (uint64_t*)A.alloc(8ull * (np + 1));
     if (!dP || !cnt || !gscan || !pbase) return set_err(c, FSIG_ENOMEM, "cedge: arena exhausted");
     FSIG_HIP(c, hipMemcpyAsync(dP, pairs.data(), sizeof(PairDesc) * np, hipMemcpyHostToDevice, c->stream));
-    switch (RG) {
-      case 8: k_cedge_count<8><<<grid_for(c, flat, FSIG_BLOCK / 8), FSIG_BLOCK, 0, c->stream>>>(dP, np, (uint32_t)flat, g->d_off, g->d_adj, mask, cnt); break;
-      case 16: k_cedge_count<16><<<grid_for(c, flat, FSIG_BLOCK / 16), FSIG_BLOCK, 0, c->stream>>>(dP, np, (uint32_t)flat, g->d_off, g->d_adj, mask, cnt); break;
-      default: k_cedge_count<64><<<grid_for(c, flat, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dP, np, (uint32_t)flat, g->d_off, g->d_adj, mask, cnt); break;
-    }
-    uint32_t* hubs = nullptr;
-    unsigned int* nhubs = nullptr;
-    const uint32_t hub_grid = (uint32_t)c->num_cus * 4;
-    if (g->max_degree > kHubDeg) {
-      hubs = (uint32_t*)A.alloc(4ull * std::max<uint64_t>(flat, 1));
-      nhubs = (unsigned int*)A.alloc(4);
-      if (!hubs || !nhubs) return set_err(c, FSIG_ENOMEM, "cedge: arena exhausted");
-      k_fill_u32<<<1, 64, 0, c->stream>>>(nhubs, 1, 0);
-      k_find_hubs<<<grid_for(c, flat, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(dP, np, (uint32_t)flat, g->d_off, hubs,
-                                                                              nhubs);
-      k_cedge_hub<false><<<hub_grid, FSIG_BLOCK, 0, c->stream>>>(dP, np, hubs, nhubs, g->d_off, g->d_adj, mask, cnt,
-                                                               nullptr, nullptr);
-    }
+    if ((rc = row_count(c, RG, dP, np, flat, g->d_off, g->d_adj, cnt))) return rc;
+    SegRows segs;  // hub rows, one wavefront per segment
+    if ((rc = seg_prepare(c, dP, np, flat, g->d_off, cnt, segs))) return rc;
+    if ((rc = seg_count(c, dP, np, flat, g->d_off, g->d_adj, cnt, segs))) return rc;
     FSIG_HIP(c, hipGetLastError());
     if ((rc = scan_u32_to_u64(c, cnt, gscan, flat))) return rc;
     uint64_t total_edges;
     if ((rc = read_u64(c, gscan + flat, &total_edges, 1))) return rc;
     uint32_t* lst_all = (uint32_t*)A.alloc(4ull * std::max<uint64_t>(total_edges, 1));
     if (!lst_all) return set_err(c, FSIG_ENOMEM, "cedge: arena exhausted for lists");
-    switch (RG) {
-      case 8: k_cedge_fill<8><<<grid_for(c, flat, FSIG_BLOCK / 8), FSIG_BLOCK, 0, c->stream>>>(dP, np, (uint32_t)flat, g->d_off, g->d_adj, mask, gscan, lst_all); break;
-      case 16: k_cedge_fill<16><<<grid_for(c, flat, FSIG_BLOCK / 16), FSIG_BLOCK, 0, c->stream>>>(dP, np, (uint32_t)flat, g->d_off, g->d_adj, mask, gscan, lst_all); break;
-      default: k_cedge_fill<64><<<grid_for(c, flat, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dP, np, (uint32_t)flat, g->d_off, g->d_adj, mask, gscan, lst_all); break;
-    }
-    if (hubs)
-      k_cedge_hub<true><<<hub_grid, FSIG_BLOCK, 0, c->stream>>>(dP, np, hubs, nhubs, g->d_off, g->d_adj, mask, nullptr,
-                                                              gscan, lst_all);
+    if ((rc = row_fill(c, RG, dP, np, flat, g->d_off, g->d_adj, gscan, lst_all))) return rc;
+    if ((rc = seg_fill(c, dP, np, flat, g->d_off, g->d_adj, gscan, lst_all, segs))) return rc;
     k_gather_u64<<<(np + 255) / 256, 256, 0, c->stream>>>(gscan, dP, np, pbase);
     FSIG_HIP(c, hipGetLastError());
     std::vector<uint64_t> hb(np);
@@ -662,11 +515,10 @@ int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_
           pv.plst[a][b] = lst_all + hb[pair_id[a][b]];
         }
     avg_list = flat ? (double)total_edges / (double)flat : 1;
-    uint64_t adj_read = 0;  // upper bound of adjacency bytes: not known on host, use lists
-    (void)adj_read;
-    // bytes: per candidate row two passes over (id, offset pair) and its
-    // adjacency, modelled by the list volume; lists and offsets written once
-    tc.done(2ull * flat * 12 + 8ull * total_edges + 4ull * total_edges + 4ull * (flat + np), flat);
+    // bytes: both passes read each row's candidate and offsets and, per
+    // adjacency entry, the entry and its mask word; the fill reads the
+    // candidate index of each kept entry and writes it; counts, offsets
+    tc.done(28ull * flat + 16ull * segs.entries + 8ull * total_edges + 4ull * (flat + np), segs.entries);
   }
   if (chain) {
     // S[v] over the candidates v of the previous position's backward

@@ -211,7 +211,8 @@ int fsig_set_profiling(fsig_ctx* ctx, int on);
 int fsig_stats_reset(fsig_ctx* ctx);
 /* launches, summed device ms (profiling on), algorithmic bytes (every
  * logical element a kernel reads or writes counted once per use, at its
- * natural width) and units processed (rows, candidates or children). */
+ * natural width) and units processed (rows, candidates or children; for
+ * FSIG_K_CEDGE the adjacency entries of the candidate rows swept). */
 int fsig_kernel_stats(const fsig_ctx* ctx, int family, uint64_t* launches, double* ms,
                       uint64_t* alg_bytes, uint64_t* units);
 /* Candidate-set sizes of the last fsig_match, per query vertex (k entries). */
