@@ -50,7 +50,7 @@ sys.path.insert(0, ROOT)
 METRIC = "embeddings/s & queries/s (batched vs per-query), p50/p99 latency, % HBM peak"
 HBM_PEAK_GBS = 8000.0
 KERNEL_OF_FAMILY = {"join_tail": "k_join_tail (fused last two levels)", "join_count": "k_join (count pass)",
-                    "join_write": "k_join (write pass)", "refine": "k_refine", "cedge": "k_cedge_count/fill",
+                    "join_write": "k_join (write pass)", "refine": "k_refine", "cedge": "k_seg_cedge + k_row_count/fill",
                     "filter": "k_ldf", "batch": "k_join_multi"}
 COUNTERS = os.path.join(ROOT, "profiles", "c2_hbm_counters.json")
 
