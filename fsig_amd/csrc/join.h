@@ -858,6 +858,17 @@ struct LeafSink {
   virtual int take(const uint32_t* rows, uint64_t n) = 0;
 };
 
+// Lanes per row of a launch over R parent rows: G, doubled while the launch
+// would leave most of the device's lanes idle (few rows, each a long walk
+// whose children and intersections the lanes split), up to Gmax.  A tuned
+// lane count is kept as it is.
+inline uint32_t widen_lanes(const fsig_ctx* c, uint32_t G, uint64_t R, uint32_t Gmax) {
+  if (c->tune.lanes_per_row) return G;
+  const uint64_t lanes = (uint64_t)c->num_cus * 32 * 64;  // 32 resident waves of 64 per CU
+  while (G < Gmax && R * G * 2 <= lanes) G <<= 1;
+  return G;
+}
+
 // Provider: supplies the level description and list source for each level:
 //   using SrcType = ...; JoinLevel level(uint32_t p); SrcType src(uint32_t p);
 template <class Provider>
@@ -929,7 +940,7 @@ class JoinDriver {
       FSIG_HIP(c_, hipMemsetAsync(res, 0, sizeof(unsigned long long), c_->stream));
       JoinIO io{rows, 0, R, nullptr, nullptr, nullptr, 0, 0, res, family_counters(c_, FSIG_K_JOIN_COUNT)};
       KTimer t(c_, FSIG_K_JOIN_COUNT);
-      if ((rc = launch_join<false>(c_, G_, L, S, io))) return rc;
+      if ((rc = launch_join<false>(c_, widen_lanes(c_, G_, R, 64), L, S, io))) return rc;
       t.done(0, 0);
       uint64_t h;
       if ((rc = read_u64(c_, res, &h, 1))) return rc;
@@ -949,7 +960,7 @@ class JoinDriver {
     if (!fused) {
       JoinIO io{rows, 0, R, cnt, nullptr, nullptr, 0, 0, nullptr, family_counters(c_, FSIG_K_JOIN_COUNT)};
       KTimer t(c_, FSIG_K_JOIN_COUNT);
-      if ((rc = launch_join<false>(c_, G_, L, S, io))) return rc;
+      if ((rc = launch_join<false>(c_, widen_lanes(c_, G_, R, 64), L, S, io))) return rc;
       t.done(0, 0);
     }
     if ((rc = scan_u32_to_u64(c_, cnt, off, R))) return rc;
@@ -1000,7 +1011,7 @@ class JoinDriver {
                 family_counters(c_, FSIG_K_JOIN_WRITE)};
       {
         KTimer t(c_, FSIG_K_JOIN_WRITE);
-        if ((rc = launch_join<true>(c_, G_, L, S, io))) return rc;
+        if ((rc = launch_join<true>(c_, widen_lanes(c_, G_, win[1] - win[0], 64), L, S, io))) return rc;
         t.done(0, 0);
       }
       if ((rc = expand(p + 1, child, hi - lo))) return rc;
@@ -1024,8 +1035,9 @@ class JoinDriver {
     // through the LDS tile, else few (rows are mostly a handful of dependent
     // gathers: more of them in flight)
     const bool needs_tile = tail_tile_words(8, L2, 0) != 0;
-    const uint32_t G = use_chain || !needs_tile ? std::min<uint32_t>(std::max<uint32_t>(G_, 4), 8)
-                                                : std::max<uint32_t>(G_, 8);
+    const uint32_t G = widen_lanes(c_, use_chain || !needs_tile ? std::min<uint32_t>(std::max<uint32_t>(G_, 4), 8)
+                                                                : std::max<uint32_t>(G_, 8),
+                                   R, use_chain ? 32 : 64);
     const TailIO io{rows, R, cnt, res, family_counters(c_, FSIG_K_JOIN_TAIL),
                     use_chain ? 0u : tail_tile_words(G, L2, c_->tune.lds_tile)};
     KTimer t(c_, FSIG_K_JOIN_TAIL);
