@@ -125,7 +125,7 @@ HIP_SYMBOLS = [
 class Tune(C.Structure):
     _fields_ = [("lanes_per_row", C.c_uint32), ("strategy", C.c_uint32), ("lds_tile", C.c_uint32),
                 ("refine_passes", C.c_uint32), ("fused_tail", C.c_uint32), ("nlf_filter", C.c_uint32),
-                ("beam_log2", C.c_uint32)]
+                ("beam_log2", C.c_uint32), ("symmetry", C.c_uint32)]
 
 
 class ServerStats(C.Structure):
@@ -445,8 +445,8 @@ class Context:
         return {"launches": la.value, "ms": ms.value, "alg_bytes": by.value, "units": un.value}
 
     def set_tune(self, lanes_per_row=0, strategy=0, lds_tile=0, refine_passes=0, fused_tail=0, nlf_filter=0,
-                 beam_log2=0):
-        t = Tune(lanes_per_row, strategy, lds_tile, refine_passes, fused_tail, nlf_filter, beam_log2)
+                 beam_log2=0, symmetry=0):
+        t = Tune(lanes_per_row, strategy, lds_tile, refine_passes, fused_tail, nlf_filter, beam_log2, symmetry)
         self.check(self._L.fsig_set_tune(self._h, C.byref(t)))
 
     def get_tune(self) -> dict:

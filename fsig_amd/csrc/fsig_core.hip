@@ -399,8 +399,19 @@ static int match_impl(fsig_ctx* c, const fsig_graph* g, uint32_t k, const int32_
     k_fill_words<<<1, 64, 0, c->stream>>>((uint32_t*)acc, 4);
     FSIG_HIP(c, hipGetLastError());
   }
-  rc = alg == FSIG_ALG_MMATCH ? run_mmatch(c, g, P, mode, max_emb, count_out, emb_out, emb_cap, acc)
-                              : run_gsi(c, g, P, mode, max_emb, count_out, emb_out, emb_cap, acc);
+  // A count counts one embedding per orbit of the query's automorphisms
+  // (symmetry conditions on the join) and multiplies by their number; a
+  // bound b then stops the search at ceil(b / |Aut|) orbits.
+  uint64_t run_bound = max_emb;
+  if (mode == FSIG_MODE_COUNT && !digest_out && c->tune.symmetry == 0 && plan_symmetry(P) && P.aut > 1 && max_emb)
+    run_bound = max_emb / P.aut + (max_emb % P.aut != 0);
+  rc = alg == FSIG_ALG_MMATCH ? run_mmatch(c, g, P, mode, run_bound, count_out, emb_out, emb_cap, acc)
+                              : run_gsi(c, g, P, mode, run_bound, count_out, emb_out, emb_cap, acc);
+  if (rc == FSIG_OK && P.aut > 1) {
+    const uint64_t orbits = *count_out;
+    *count_out = orbits > UINT64_MAX / P.aut ? UINT64_MAX : orbits * P.aut;
+    if (max_emb) *count_out = std::min(*count_out, max_emb);
+  }
   if (rc == FSIG_OK && acc) rc = read_u64(c, acc, digest_out, 2);
   c->last_peak = c->arena.peak - mk;
   c->arena.release(mk);
@@ -601,6 +612,7 @@ int fsig_set_tune(fsig_ctx* c, const fsig_tune* t) {
   if (t->fused_tail > 1) return set_err(c, FSIG_EINVAL, "tune: fused_tail");
   if (t->nlf_filter > 1) return set_err(c, FSIG_EINVAL, "tune: nlf_filter");
   if (t->beam_log2 > 40) return set_err(c, FSIG_EINVAL, "tune: beam_log2");
+  if (t->symmetry > 1) return set_err(c, FSIG_EINVAL, "tune: symmetry");
   c->tune = *t;
   return FSIG_OK;
 }

@@ -84,6 +84,8 @@ struct GsiProvider {
         L.cfsame |= 1u << L.ncf;  // rows hold data vertex ids
         L.ncf++;
       }
+      if ((P->sym_lt[u] >> w) & 1u) L.gtmask |= 1u << c;  // symmetry: phi(u) < phi(w)
+      if ((P->sym_lt[w] >> u) & 1u) L.ltmask |= 1u << c;  // phi(w) < phi(u)
     }
     return L;
   }

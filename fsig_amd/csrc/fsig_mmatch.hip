@@ -202,6 +202,11 @@ struct MMatchProvider {
         }
       }
     }
+    for (uint32_t c = 0; c < p; c++) {  // symmetry conditions against earlier columns
+      const uint32_t u = P->order[c];
+      if ((P->sym_lt[u] >> w) & 1u) L.gtmask |= 1u << c;  // phi(u) < phi(w)
+      if ((P->sym_lt[w] >> u) & 1u) L.ltmask |= 1u << c;  // phi(w) < phi(u)
+    }
     return L;
   }
   CandEdgeSrc src(uint32_t p) const {

@@ -136,11 +136,18 @@ struct QueryPlan {
   // per label among the representative's neighbours
   uint32_t nlf_n[MAXQ];
   uint32_t nlf_req[MAXQ][MAXQ];
+  // symmetry breaking (count mode): bit b of sym_lt[a] asks phi(a) < phi(b);
+  // the count is aut times the number of embeddings that satisfy all of them
+  uint64_t aut = 1;
+  uint32_t sym_lt[MAXQ] = {0};
 };
 
 int plan_classes(QueryPlan& P, uint32_t k, const int32_t* qlabels, const uint32_t* qoff,
                  const uint32_t* qadj, const uint32_t* qsig, int alg, std::string& err);
 void plan_order_mmatch(QueryPlan& P, const uint32_t* class_size);
+// Fills aut and sym_lt (after plan_classes); false when the automorphism
+// search ran past its step budget (then aut = 1 and no conditions).
+bool plan_symmetry(QueryPlan& P);
 void plan_order_gsi(QueryPlan& P, const uint32_t* class_size);
 
 // ---------------------------------------------------------------------------

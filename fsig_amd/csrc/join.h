@@ -45,9 +45,54 @@ struct JoinLevel {
   // data vertex -> candidate index of w (n words; meaningful where
   // wcand[wpos[v]] == v), or null: cand_index then binary-searches wcand
   const uint32_t* wpos;
+  // symmetry breaking (count mode): a child x must exceed the row word of
+  // every column in gtmask and stay below that of every column in ltmask
+  // (columns of w's class: the words compare as the data vertex ids do)
+  uint32_t gtmask, ltmask;
 };
 
 constexpr uint32_t kNone = 0xffffffffu;
+
+// First index of the ascending a[0..n) whose element is >= x.
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__ a, uint32_t n, uint32_t x) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Bounds [lo, hi) that the symmetry conditions of a level leave for the
+// child of a row (rv: the row, G lanes; uniform over the group).
+template <int G>
+__device__ __forceinline__ uint32_t group_col(const uint32_t (&rv)[(MAXQ + G - 1) / G], uint32_t c);
+
+template <int G>
+__device__ __forceinline__ void sym_bounds(uint32_t gtmask, uint32_t ltmask, const uint32_t (&rv)[(MAXQ + G - 1) / G],
+                                           uint32_t& lo, uint32_t& hi) {
+  lo = 0;
+  hi = kNone;
+  for (uint32_t m = gtmask; m; m &= m - 1) lo = max(lo, group_col<G>(rv, __builtin_ctz(m)) + 1);
+  for (uint32_t m = ltmask; m; m &= m - 1) hi = min(hi, group_col<G>(rv, __builtin_ctz(m)));
+}
+
+// Driver positions [ta, tb) whose elements fall in [lo, hi): the driver is
+// the ascending list dl, or (dl null) the identity over [0, dlen), or wcand.
+__device__ __forceinline__ void driver_window(const uint32_t* dl, const uint32_t* wcand, bool ids, uint32_t dlen,
+                                              uint32_t lo, uint32_t hi, uint32_t& ta, uint32_t& tb) {
+  if (dl) {
+    ta = lo ? lower_bound_u32(dl, dlen, lo) : 0;
+    tb = hi != kNone ? lower_bound_u32(dl, dlen, hi) : dlen;
+  } else if (ids) {
+    ta = lo ? lower_bound_u32(wcand, dlen, lo) : 0;
+    tb = hi != kNone ? lower_bound_u32(wcand, dlen, hi) : dlen;
+  } else {
+    ta = min(lo, dlen);
+    tb = min(hi, dlen);
+  }
+  if (tb < ta) tb = ta;
+}
 
 // Candidate index of data vertex id in w's class, or kNone when id is no
 // candidate of w: one or two gathers through wpos, else a binary search.
@@ -179,6 +224,13 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join(JoinLevel L, Src S, JoinIO 
       if (io.off[r + 1] == child0) continue;  // no children (or counted by a fused tail)
     }
     const uint32_t* dl = L.nb ? S.list(d) + sh_s[gw][d] : nullptr;
+    // symmetry conditions: the children lie in [lo, hi), driver positions
+    // [ta, tb)
+    uint32_t lo = 0, hi = kNone, ta = 0, tb = dlen;
+    if (L.gtmask | L.ltmask) {
+      sym_bounds<G>(L.gtmask, L.ltmask, rv, lo, hi);
+      driver_window(dl, L.wcand, L.range_ids != 0, dlen, lo, hi, ta, tb);
+    }
     // Count pass with one backward list of candidates only: the children are
     // the list minus the earlier vertices of the same label that occur in
     // it, found by binary search (one lane per conflict).
@@ -191,16 +243,16 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join(JoinLevel L, Src S, JoinIO 
         if (j < L.ncf) {
           uint32_t target = sh_cf[gw][j];
           if (!((L.cfsame >> j) & 1u)) target = cand_index(L, target);  // data vertex id -> candidate index of w
-          in = target != kNone && sorted_contains(dl, dlen, target);
+          in = target != kNone && target >= lo && target < hi && sorted_contains(dl, dlen, target);
         }
         hit += __popcll(__ballot(in) & gmask);
       }
-      base = dlen - hit;
+      base = (tb - ta) - hit;
       elems = 0;  // the list itself is not read
     }
-    for (uint32_t t0 = 0; !direct && t0 < dlen; t0 += G) {
+    for (uint32_t t0 = ta; !direct && t0 < tb; t0 += G) {
       const uint32_t t = t0 + gl;
-      bool ok = t < dlen;
+      bool ok = t < tb;
       uint32_t x = 0;
       if (ok) x = L.nb ? dl[t] : (L.range_ids ? L.wcand[t] : t);
       for (uint32_t i = 0; i < L.nb && ok; i++) {
@@ -353,25 +405,40 @@ __device__ __forceinline__ uint64_t tail_row(const JoinLevel& L1, const CandEdge
   }
   uint64_t lbytes = gl == 0 ? 4ull * p + 8ull * (L1.nb + ns) + 4ull * el : 0;
   const uint32_t* dl = L1.nb ? S1.lst[d] + sh.s1[d] : nullptr;
+  // symmetry conditions: children x in [lo1, hi1) (driver positions [ta1,
+  // tb1)); leaves y in [lo2, hi2), and above (g2p) or below (l2p) their own
+  // x
+  const uint32_t pbit = 1u << p;
+  const bool g2p = (L2.gtmask & pbit) != 0, l2p = (L2.ltmask & pbit) != 0;
+  uint32_t lo1 = 0, hi1 = kNone, lo2 = 0, hi2 = kNone, ta1 = 0, tb1 = dlen;
+  if (L1.gtmask | L1.ltmask) {
+    sym_bounds<G>(L1.gtmask, L1.ltmask, rv, lo1, hi1);
+    driver_window(dl, nullptr, false, dlen, lo1, hi1, ta1, tb1);
+  }
+  if ((L2.gtmask | L2.ltmask) & ~pbit) sym_bounds<G>(L2.gtmask & ~pbit, L2.ltmask & ~pbit, rv, lo2, hi2);
+  const bool leaf_bounds = lo2 != 0 || hi2 != kNone;
   if (own == kNone) {
     // No own list: every child has the same leaves I except its own image.
     // |I| once per parent (one shared list: its length minus the conflicts
     // found in it; more: a counting walk), then one membership test per child.
     uint64_t isz = 0;
     const uint32_t* dl2 = S2.lst[ds] + sh.s2[ds];
+    uint32_t a2 = 0, b2 = dslen;
+    if (leaf_bounds) driver_window(dl2, nullptr, false, dslen, lo2, hi2, a2, b2);
     if (ns == 1) {
       uint32_t hit = 0;
       for (uint32_t j0 = 0; j0 < L2.ncf; j0 += G) {
         const uint32_t j = j0 + gl;
-        const bool in = j < L2.ncf && sh.cf2[j] != kNone && sorted_contains(dl2, dslen, sh.cf2[j]);
+        const bool in = j < L2.ncf && sh.cf2[j] != kNone && sh.cf2[j] >= lo2 && sh.cf2[j] < hi2 &&
+                        sorted_contains(dl2, dslen, sh.cf2[j]);
         hit += (uint32_t)__popcll(__ballot(in) & gmask);
       }
-      isz = dslen - hit;
+      isz = (b2 - a2) - hit;
       if (gl == 0) lbytes -= 4ull * dslen;  // the list itself is not read
     } else {
-      for (uint32_t t0 = 0; t0 < dslen; t0 += G) {
+      for (uint32_t t0 = a2; t0 < b2; t0 += G) {
         const uint32_t t = t0 + gl;
-        bool ok = t < dslen;
+        bool ok = t < b2;
         const uint32_t y = ok ? dl2[t] : 0u;
         for (uint32_t i = 0; i < L2.nb && ok; i++) {
           if (i == ds) continue;
@@ -392,19 +459,19 @@ __device__ __forceinline__ uint64_t tail_row(const JoinLevel& L1, const CandEdge
         if (j < L1.ncf) {
           uint32_t x = sh.cf1[j];
           if (!((L1.cfsame >> j) & 1u)) x = cand_index(L1, x);  // a data vertex id
-          in = x != kNone && sorted_contains(dl, dlen, x);
+          in = x != kNone && x >= lo1 && x < hi1 && sorted_contains(dl, dlen, x);
         }
         hit += (uint32_t)__popcll(__ballot(in) & gmask);
       }
-      children = dlen - hit;
+      children = (tb1 - ta1) - hit;
       if (gl == 0) lbytes += 4ull * L1.ncf * (2 + (32 - __builtin_clz(dlen | 1))) - 4ull * dlen;  // list not read
       bytes += lbytes;
       return gl == 0 ? children * isz : 0;
     }
     uint64_t kids = 0, leaves = 0;
-    for (uint32_t t0 = 0; t0 < dlen; t0 += G) {
+    for (uint32_t t0 = ta1; t0 < tb1; t0 += G) {
       const uint32_t t = t0 + gl;
-      bool ok = t < dlen;
+      bool ok = t < tb1;
       uint32_t x = 0;
       if (ok) x = L1.nb ? dl[t] : t;
       for (uint32_t i = 0; i < L1.nb && ok; i++) {
@@ -421,7 +488,7 @@ __device__ __forceinline__ uint64_t tail_row(const JoinLevel& L1, const CandEdge
         if (ccf != kNone) {  // x's image is a leaf candidate unless some shared list lacks it
           uint32_t cx = x;
           if (!((L2.cfsame >> ccf) & 1u)) cx = cand_index(L2, L2.cfcand[ccf][x]);
-          bool in = cx != kNone;
+          bool in = cx != kNone && cx >= lo2 && cx < hi2;
           for (uint32_t i = 0; i < L2.nb && in; i++) in = sorted_contains(S2.lst[i] + sh.s2[i], sh.n2[i], cx);
           if (in) cnt--;
           lbytes += 4ull * ns * (32 - __builtin_clz(dslen | 1));
@@ -443,9 +510,11 @@ __device__ __forceinline__ uint64_t tail_row(const JoinLevel& L1, const CandEdge
   uint32_t isz = 0;
   if (ns > 0) {
     const uint32_t* dl2 = S2.lst[ds] + sh.s2[ds];
-    for (uint32_t t0 = 0; t0 < dslen; t0 += G) {
+    uint32_t a2 = 0, b2 = dslen;
+    if (leaf_bounds) driver_window(dl2, nullptr, false, dslen, lo2, hi2, a2, b2);
+    for (uint32_t t0 = a2; t0 < b2; t0 += G) {
       const uint32_t t = t0 + gl;
-      bool ok = t < dslen;
+      bool ok = t < b2;
       const uint32_t y = ok ? dl2[t] : 0u;
       for (uint32_t i = 0; i < L2.nb && ok; i++) {
         if (i == ds || i == own) continue;
@@ -460,12 +529,12 @@ __device__ __forceinline__ uint64_t tail_row(const JoinLevel& L1, const CandEdge
     }
     lds_wave_sync();
     // many children, each with a large I to intersect: spread them instead
-    defer_row = defer_row || (dlen > 16u * G && isz > 4u * G);
+    defer_row = defer_row || (tb1 - ta1 > 16u * G && isz > 4u * G);
   }
   uint64_t kids = 0, leaves = 0;
-  for (uint32_t t0 = 0; t0 < dlen; t0 += G) {
+  for (uint32_t t0 = ta1; t0 < tb1; t0 += G) {
     const uint32_t t = t0 + gl;
-    bool ok = t < dlen;
+    bool ok = t < tb1;
     uint32_t x = 0;
     if (ok) x = L1.nb ? dl[t] : t;
     for (uint32_t i = 0; i < L1.nb && ok; i++) {
@@ -484,9 +553,13 @@ __device__ __forceinline__ uint64_t tail_row(const JoinLevel& L1, const CandEdge
       if (ok) {
         const uint32_t s = S2.off[own][x], n = S2.off[own][x + 1] - s;
         const uint32_t* l = S2.lst[own] + s;
-        uint64_t cnt = n;
+        uint32_t lox = lo2, hix = hi2, a = 0, b = n;
+        if (g2p) lox = max(lox, x + 1);
+        if (l2p) hix = min(hix, x);
+        if (lox != 0 || hix != kNone) driver_window(l, nullptr, false, n, lox, hix, a, b);
+        uint64_t cnt = b - a;
         for (uint32_t j = 0; j < L2.ncf; j++)
-          if (sh.cf2[j] != kNone && sorted_contains(l, n, sh.cf2[j])) cnt--;
+          if (sh.cf2[j] != kNone && sh.cf2[j] >= lox && sh.cf2[j] < hix && sorted_contains(l, n, sh.cf2[j])) cnt--;
         leaves += cnt;
         lbytes += 8 + 4ull * L2.ncf * (32 - __builtin_clz(n | 1));
       }
@@ -499,18 +572,20 @@ __device__ __forceinline__ uint64_t tail_row(const JoinLevel& L1, const CandEdge
       const uint32_t xx = sh.slot[q];
       const uint32_t s = S2.off[own][xx], n = S2.off[own][xx + 1] - s;
       const uint32_t* l = S2.lst[own] + s;
+      // leaves above (g2p) or below (l2p) the child itself
+      const uint32_t lox = g2p ? xx + 1 : 0, hix = l2p ? xx : kNone;
       uint32_t got = 0;
       if (isz <= n) {
         for (uint32_t e0 = 0; e0 < isz; e0 += G) {
           const uint32_t e = e0 + gl;
-          const bool in = e < isz && sorted_contains(l, n, sh.tile[e]);
+          const bool in = e < isz && sh.tile[e] >= lox && sh.tile[e] < hix && sorted_contains(l, n, sh.tile[e]);
           got += (uint32_t)__popcll(__ballot(in) & gmask);
         }
         if (gl == 0) lbytes += 8 + 4ull * isz * (32 - __builtin_clz(n | 1));
       } else {
         for (uint32_t e0 = 0; e0 < n; e0 += G) {
           const uint32_t e = e0 + gl;
-          const bool in = e < n && sorted_contains(sh.tile, isz, l[e]);
+          const bool in = e < n && l[e] >= lox && l[e] < hix && sorted_contains(sh.tile, isz, l[e]);
           got += (uint32_t)__popcll(__ballot(in) & gmask);
         }
         if (gl == 0) lbytes += 8 + 4ull * n;
@@ -601,8 +676,18 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join_tail(JoinLevel L1, CandEdge
 }
 
 // Whether the fused tail applies to leaf level L2: it needs a backward list,
-// and its lists and conflicts must fit the group arrays.
-inline bool tail_applies(const JoinLevel& L2) { return L2.nb >= 1 && L2.nb <= kTailMax && L2.ncf <= kTailMax; }
+// its lists and conflicts must fit the group arrays, and a symmetry
+// condition between leaf and child needs the leaf adjacent to the child.
+inline bool tail_applies(const JoinLevel& L2) {
+  if (!(L2.nb >= 1 && L2.nb <= kTailMax && L2.ncf <= kTailMax)) return false;
+  // a symmetry condition between the leaf and the child needs the child's
+  // own list to bound
+  const uint32_t pbit = 1u << (L2.p - 1);
+  if (!((L2.gtmask | L2.ltmask) & pbit)) return true;
+  for (uint32_t i = 0; i < L2.nb; i++)
+    if (L2.bcol[i] == L2.p - 1) return true;
+  return false;
+}
 
 // Tile words per group the tail of L2 needs: only a leaf with both shared
 // lists (parent columns) and an own list (column p) intersects in LDS.
@@ -789,7 +874,8 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_join_tail_chain(JoinLevel L1, Ca
 
 // Whether the chain tail applies to levels p (L1) and p+1 (L2).
 inline bool chain_tail_applies(const JoinLevel& L1, const JoinLevel& L2) {
-  return L2.nb == 1 && L2.bcol[0] == L1.p && L1.nb == 1 && L1.ncf <= kTailMax && L2.ncf <= kTailMax;
+  return L2.nb == 1 && L2.bcol[0] == L1.p && L1.nb == 1 && L1.ncf <= kTailMax && L2.ncf <= kTailMax &&
+         !(L1.gtmask | L1.ltmask | L2.gtmask | L2.ltmask);  // its sums cover every child and leaf
 }
 
 inline int launch_join_tail_chain(fsig_ctx* c, uint32_t G, const JoinLevel& L1, const CandEdgeSrc& S1,

@@ -179,3 +179,92 @@ void plan_order_gsi(QueryPlan& P, const uint32_t* class_size) {
       if ((P.adjm[best] >> w) & 1) score[w] *= 0.9f;
   }
 }
+
+// ---------------------------------------------------------------------------
+// Symmetry breaking for counts (Grochow and Kellis, "Network motif discovery
+// using subgraph enumeration and symmetry-breaking", RECOMB 2007).  An
+// automorphism s of the query (a permutation of its vertices that keeps
+// labels and edges) turns every embedding phi into the embedding phi o s,
+// and distinct automorphisms give distinct embeddings, so the embeddings
+// fall into orbits of exactly |Aut| each.  Walking down a stabiliser chain
+// (take a vertex v of largest orbit under the automorphisms that fix the
+// vertices taken so far, ask phi(v) < phi(w) for the other members w of that
+// orbit, then fix v) yields conditions that exactly one embedding of every
+// orbit satisfies, and |Aut| is the product of the orbit sizes taken.
+// Automorphic vertices share a refinement class, hence a candidate array,
+// so a condition compares two row words of the same class.
+
+namespace {
+
+struct AutSearch {
+  const QueryPlan& P;
+  uint32_t img[MAXQ];
+  uint64_t steps = 0;
+  static constexpr uint64_t kBudget = 1u << 22;
+  explicit AutSearch(const QueryPlan& p) : P(p) {}
+
+  // Extends the partial permutation img[0..u) (images in `used`) to a full
+  // automorphism; pre[u] != FSIG_INVALID prescribes u's image.
+  bool extend(uint32_t u, uint32_t used, const uint32_t* pre) {
+    if (u == P.k) return true;
+    if (++steps > kBudget) return false;
+    for (uint32_t t = 0; t < P.k; t++) {
+      if ((used >> t) & 1u) continue;
+      if (pre[u] != FSIG_INVALID && t != pre[u]) continue;
+      if (P.cls[t] != P.cls[u]) continue;  // classes are kept by automorphisms
+      bool ok = true;
+      for (uint32_t a = 0; a < u && ok; a++)
+        ok = (((P.adjm[u] >> a) & 1u) == ((P.adjm[t] >> img[a]) & 1u));
+      if (!ok) continue;
+      img[u] = t;
+      if (extend(u + 1, used | (1u << t), pre)) return true;
+      if (steps > kBudget) return false;
+    }
+    return false;
+  }
+};
+
+}  // namespace
+
+bool plan_symmetry(QueryPlan& P) {
+  const uint32_t k = P.k;
+  P.aut = 1;
+  for (uint32_t u = 0; u < MAXQ; u++) P.sym_lt[u] = 0;
+  AutSearch S(P);
+  uint32_t fixed = 0;
+  uint32_t lt[MAXQ] = {0};
+  uint64_t aut = 1;
+  for (;;) {
+    // orbits under the automorphisms fixing `fixed` pointwise
+    uint32_t orbit[MAXQ];
+    for (uint32_t v = 0; v < k; v++) orbit[v] = 1u << v;
+    uint32_t pre[MAXQ];
+    for (uint32_t v = 0; v < k; v++) {
+      if ((fixed >> v) & 1u) continue;
+      for (uint32_t w = v + 1; w < k; w++) {
+        if (((fixed >> w) & 1u) || P.cls[w] != P.cls[v] || ((orbit[v] >> w) & 1u)) continue;
+        for (uint32_t x = 0; x < k; x++) pre[x] = ((fixed >> x) & 1u) ? x : FSIG_INVALID;
+        pre[v] = w;
+        const bool found = S.extend(0, 0, pre);
+        if (S.steps > AutSearch::kBudget) return false;
+        if (!found) continue;
+        const uint32_t merged = orbit[v] | orbit[w];
+        for (uint32_t x = 0; x < k; x++)
+          if ((merged >> x) & 1u) orbit[x] = merged;
+      }
+    }
+    uint32_t best = FSIG_INVALID, best_n = 1;
+    for (uint32_t v = 0; v < k; v++) {
+      if ((fixed >> v) & 1u) continue;
+      const uint32_t n = (uint32_t)__builtin_popcount(orbit[v]);
+      if (n > best_n) { best_n = n; best = v; }
+    }
+    if (best == FSIG_INVALID) break;
+    lt[best] |= orbit[best] & ~(1u << best);
+    aut *= best_n;
+    fixed |= 1u << best;
+  }
+  P.aut = aut;
+  for (uint32_t u = 0; u < k; u++) P.sym_lt[u] = lt[u];
+  return true;
+}

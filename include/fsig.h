@@ -247,6 +247,12 @@ typedef struct fsig_tune {
   uint32_t beam_log2;     /* fsig_match_batch_bounded: beam of a bounded query
                              in the lockstep join, 2^beam_log2 children
                              (0: 16; 1..40) */
+  uint32_t symmetry;      /* 0: a count-mode fsig_match counts one embedding
+                             per orbit of the query's automorphism group
+                             (symmetry-breaking conditions phi(a) < phi(b)
+                             on the join, Grochow and Kellis 2007) and
+                             multiplies by the group's order; 1: every
+                             embedding.  Same result either way. */
 } fsig_tune;
 int fsig_set_tune(fsig_ctx* ctx, const fsig_tune* t);
 int fsig_get_tune(const fsig_ctx* ctx, fsig_tune* t);

@@ -2,7 +2,8 @@
 with g++ under AddressSanitizer and UndefinedBehaviorSanitizer (host code
 only): equitable classes with exact thresholds and neighbour-label
 requirements, signature classes for alg 2, join orders that are
-permutations (tests/plan_check.cc)."""
+permutations, symmetry-breaking conditions against brute-force
+automorphism groups (tests/plan_check.cc)."""
 import os
 import shutil
 import subprocess
