@@ -6,7 +6,10 @@
       full embedding sets (every embedding enumerated on the device and on
       the host).
   C3  unlabeled R-MAT (0.57, 0.19, 0.19), 2^22 V / 35M E: the triangle count
-      equals the oracle's (orientation count); the first 10^8 embeddings of
+      equals the oracle's (orientation count); the 4-clique count is the same
+      through the fused last two levels and level by level (both with the
+      symmetry conditions), and so is the 4-cycle count on the scale-20
+      graph of the same generator and density; the first 10^8 embeddings of
       the triangle, 4-cycle, 4-clique and house are each checked row by row
       (labels, injectivity, every query edge) and for distinctness.
 
@@ -70,6 +73,39 @@ def test_c3_triangle_count_equals_oracle(c3):
     want = 6 * O.triangles(off, adj, threads=THREADS)
     say(f"C3 triangle: oracle {want}; device count")
     assert F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT).count == want
+
+
+def _count_plan(ctx, dg, q, fused):
+    ctx.set_tune(fused_tail=0 if fused else 1)
+    try:
+        return F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT).count
+    finally:
+        ctx.set_tune()
+
+
+def test_c3_clique4_count_equal_across_plans(c3, big_ctx):
+    g, dg = c3
+    q = dict(W.c3_queries())["clique4"]
+    say("C3 4-clique: fused last two levels")
+    a = _count_plan(big_ctx, dg, q, True)
+    say(f"C3 4-clique: {a}; level by level")
+    b = _count_plan(big_ctx, dg, q, False)
+    say(f"C3 4-clique: {b}")
+    assert a == b and a > 0
+
+
+def test_c3_scale20_cycle4_count_equal_across_plans(big_ctx):
+    say("R-MAT scale 20 graph")
+    g = W.c3_graph(20, 35_000_000 // 4)
+    dg = big_ctx.upload(g)
+    q = dict(W.c3_queries())["cycle4"]
+    say("scale 20 4-cycle: fused last two levels")
+    a = _count_plan(big_ctx, dg, q, True)
+    say(f"scale 20 4-cycle: {a}; level by level")
+    b = _count_plan(big_ctx, dg, q, False)
+    say(f"scale 20 4-cycle: {b}")
+    dg.free()
+    assert a == b and a > 0
 
 
 @pytest.mark.parametrize("name", ["triangle", "cycle4", "clique4", "house"])
