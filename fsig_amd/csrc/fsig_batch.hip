@@ -29,7 +29,6 @@
 #include <cstring>
 #include <functional>
 #include <map>
-#include <mutex>
 
 #include "common.h"
 #include "device.h"
@@ -1175,12 +1174,6 @@ struct StageClock {
   }
 };
 
-// Per-context cache of tuning decisions, keyed by a coarse batch shape.
-// Contexts may run batches from several host threads at once (the served
-// loop does), so the cache is locked.
-static std::map<std::pair<const fsig_ctx*, uint64_t>, TuneChoice> g_tuned;
-static std::mutex g_tuned_mu;
-
 // Beam of a bounded query in the lockstep join: the children of its first
 // parent rows up to this many (fsig_tune.beam_log2 overrides).  Queries with
 // more embeddings than their bound usually find them inside the beam, all in
@@ -2017,21 +2010,44 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
     const DfsArgs da{dQ, dL, dsub, (uint32_t)sub.size(), qu, dtasks, ready, dcap, stacks, stack_words, dcounts, stats};
     int rc2;
     if (capture) {
-      // queue reset + persistent kernel, captured once and launched as a graph
+      // queue reset + persistent kernel as one graph.  The instantiated graph
+      // of this engine shape is kept by the context and updated from each
+      // new capture (its kernel arguments change with the batch); a capture
+      // whose topology no longer matches is instantiated afresh.
       hipGraph_t graph;
-      hipGraphExec_t exec;
       FSIG_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       rc2 = launch_dfs(c, tc, grid, c->stream, da);
       hipError_t ce = hipStreamEndCapture(c->stream, &graph);
       if (rc2) { A.release(mk); return rc2; }
       if (ce != hipSuccess) { A.release(mk); return set_err(c, FSIG_EHIP, std::string("batch: capture: ") + hipGetErrorString(ce)); }
-      FSIG_HIP(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      const uint64_t ekey = ((uint64_t)tc.G << 48) ^ ((uint64_t)tc.lds << 47) ^ ((uint64_t)tc.tile << 32) ^ grid;
+      auto it = c->batch_exec.find(ekey);
+      bool ready = false;
+      if (it != c->batch_exec.end()) {
+        hipGraphExecUpdateResult ur;
+        hipGraphNode_t bad = nullptr;
+        ready = hipGraphExecUpdate(it->second, graph, &bad, &ur) == hipSuccess && ur == hipGraphExecUpdateSuccess;
+        if (!ready) {
+          (void)hipGetLastError();
+          (void)hipGraphExecDestroy(it->second);
+          c->batch_exec.erase(it);
+        }
+      }
+      if (!ready) {
+        hipGraphExec_t exec;
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+          (void)hipGraphDestroy(graph);
+          A.release(mk);
+          return set_err(c, FSIG_EHIP, std::string("batch: instantiate: ") + hipGetErrorString(ie));
+        }
+        it = c->batch_exec.emplace(ekey, exec).first;
+      }
+      (void)hipGraphDestroy(graph);
       FSIG_HIP(c, hipEventRecord(c->ev_a, c->stream));
-      FSIG_HIP(c, hipGraphLaunch(exec, c->stream));
+      FSIG_HIP(c, hipGraphLaunch(it->second, c->stream));
       FSIG_HIP(c, hipEventRecord(c->ev_b, c->stream));
       FSIG_HIP(c, hipEventSynchronize(c->ev_b));
-      (void)hipGraphExecDestroy(exec);
-      (void)hipGraphDestroy(graph);
     } else {
       FSIG_HIP(c, hipEventRecord(c->ev_a, c->stream));
       if ((rc2 = launch_dfs(c, tc, grid, c->stream, da))) { A.release(mk); return rc2; }
@@ -2050,7 +2066,9 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
   std::vector<uint32_t> all_q;
   for (uint32_t i = 0; i < nq; i++)
     if (active[i]) all_q.push_back(i);
-  const uint64_t shape = ((uint64_t)nq << 40) ^ ((uint64_t)g->n << 8) ^ (uint64_t)stack_words;
+  // batch shape of the tuning cache (with the graph's pointer and size, so
+  // a graph uploaded later at a freed graph's address does not inherit it)
+  const uint64_t shape = ((uint64_t)nq << 40) ^ ((uint64_t)g->n << 8) ^ (uint64_t)stack_words ^ (g->nnz << 20);
   TuneChoice best{16, false, 0, true};
   if (any_bound || c->tune.strategy == 3) {
     // bounded batches settle queries inside the lockstep join, which timing
@@ -2062,9 +2080,11 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
   } else {
     bool cached = false;
     {
-      std::lock_guard<std::mutex> lk(g_tuned_mu);
-      auto it = g_tuned.find({c, shape});
-      if (it != g_tuned.end()) { best = it->second; cached = true; }
+      auto it = c->batch_tuned.find({g, shape});
+      if (it != c->batch_tuned.end()) {
+        best = TuneChoice{it->second[0], it->second[1] != 0, it->second[2], it->second[3] != 0};
+        cached = true;
+      }
     }
     if (!cached) {
       // The lockstep shapes are timed on the whole batch (a few heavy
@@ -2099,8 +2119,7 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
         if (rc) return rc;
         if (ms * scale < best_ms) { best_ms = (float)(ms * scale); best = tc; }
       }
-      std::lock_guard<std::mutex> lk(g_tuned_mu);
-      g_tuned[{c, shape}] = best;
+      c->batch_tuned[{g, shape}] = {best.G, best.lds ? 1u : 0u, best.tile, best.lockstep ? 1u : 0u};
     }
   }
   clk.mark("autotune");

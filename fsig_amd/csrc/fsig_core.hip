@@ -205,6 +205,7 @@ void fsig_ctx_destroy(fsig_ctx* c) {
   }
   if (c->ev_a) (void)hipEventDestroy(c->ev_a);
   if (c->ev_b) (void)hipEventDestroy(c->ev_b);
+  for (auto& e : c->batch_exec) (void)hipGraphExecDestroy(e.second);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -3,7 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -82,6 +84,12 @@ struct fsig_ctx {
   uint32_t last_tune_G = 0;
   uint32_t last_tune_lds = 0;
   uint32_t last_tune_engine = 0;  // 1 lockstep batched BFS, 2 persistent walk
+  // the batched engine's autotuner decisions, per graph and batch shape
+  // ({lanes, lds, tile, lockstep}), and its persistent walk's instantiated
+  // graphs, per batch shape (updated from each new capture, destroyed with
+  // the context)
+  std::map<std::pair<const fsig_graph*, uint64_t>, std::array<uint32_t, 4>> batch_tuned;
+  std::map<uint64_t, hipGraphExec_t> batch_exec;
 };
 
 struct UnitMeta {

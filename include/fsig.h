@@ -174,10 +174,12 @@ int fsig_match_batch(fsig_ctx* ctx, const fsig_graph* g, uint32_t nq, const uint
  * the lockstep batch; when its next level would exceed the beam (2^16
  * children, fsig_tune.beam_log2) only its first parent rows, whose children
  * start within the beam, go on, and the count found from then on is a lower
- * bound that settles the query once it reaches the bound.  A beamed query
- * that ends below its bound is counted again in full by fsig_match's
- * memory-blocked depth-first driver, which stops at the bound.  That driver
- * honours the context's work limit (fsig_set_work_limit): a
+ * bound that settles the query once it reaches the bound.  The cut rows (a
+ * suffix of the query's rows at that level) stay in the batch's tables; a
+ * beamed query that ends below its bound has their subtrees counted by
+ * fsig_match's memory-blocked depth-first driver, starting from those rows
+ * and stopping at what the bound leaves.  That driver honours the context's
+ * work limit (fsig_set_work_limit): a
  * query over it is refused, status_out[i] = FSIG_ELIMIT and counts_out[i]
  * = 0, while the other queries complete (status_out NULL: the call then
  * returns FSIG_ELIMIT).  No autotuning (lockstep with lanes per query).  No

@@ -67,6 +67,29 @@ def test_batch_configurations_agree(gpu_ctx, lanes, strategy):
         gpu_ctx.set_tune()
 
 
+def test_persistent_graph_replays_after_updates(gpu_ctx):
+    """The persistent walk keeps its instantiated graph per engine shape and
+    updates it from each new capture: replays with other batches (other
+    tables, queue sizes and counts), eager plain matches and
+    synchronisations in between, each replay equal to the plain counts."""
+    g = F.HostGraph.chung_lu(15000, 90000, 2.3, 17).set_labels("uniform", 4, seed=18)
+    dg = gpu_ctx.upload(g)
+    batches = [_bounded(dg, [q for _, q in W.walk_queries(g, n, 3, 8, seed)], 5_000_000)
+               for n, seed in ((24, 21), (10, 22), (31, 23))]
+    try:
+        gpu_ctx.set_tune(lanes_per_row=16, strategy=1)
+        for rep in range(2):
+            for qs in batches:
+                counts, _ = F.match_batch(dg, [q for q, _ in qs])
+                assert counts.tolist() == [c for _, c in qs], rep
+                assert F.last_batch_tune(gpu_ctx)[2] == "persistent"
+                q0, c0 = qs[0]
+                assert F.match(dg, q0, F.ALG_MMATCH, F.MODE_COUNT).count == c0  # eager work between replays
+                gpu_ctx.synchronize()
+    finally:
+        gpu_ctx.set_tune()
+
+
 def test_batch_autotuner_and_repeat(gpu_ctx):
     g = F.HostGraph.chung_lu(30000, 200000, 2.3, 12).set_labels("zipf", 16, 1.2, 13)
     dg = gpu_ctx.upload(g)
