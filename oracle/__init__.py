@@ -50,6 +50,9 @@ def lib():
                                    _u32p, C.c_uint64, _u64p]
         L.oracle_match_limit.restype = C.c_uint64
         L.oracle_match_limit.argtypes = L.oracle_match.argtypes + [C.c_uint64, C.POINTER(C.c_int)]
+        L.oracle_match_budget.restype = C.c_uint64
+        L.oracle_match_budget.argtypes = [C.c_uint32, _i32p, _u32p, C.c_uint32, _u32p, _u32p, _i32p, C.c_uint32,
+                                          C.c_uint64, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.oracle_match_digest.restype = C.c_uint64
         L.oracle_match_digest.argtypes = [C.c_uint32, _i32p, _u32p, C.c_uint32, _u32p, _u32p, _i32p, C.c_uint32,
                                           _u64p]
@@ -150,6 +153,19 @@ def match_partial(qlabels, qadjm, off, adj, labels, threads: int = 1, cap: int =
     cnt = lib().oracle_match_limit(k, _p(ql, _i32p), _p(qa), len(labels), _p(off), _p(adj), _p(labels, _i32p),
                                    threads, _p(rows), cap, C.byref(written), limit, C.byref(stopped))
     return int(cnt), (rows[: written.value] if cap else None), bool(stopped.value)
+
+
+def match_budget(qlabels, qadjm, off, adj, labels, threads: int = 1, limit: int = 0, max_steps: int = 0):
+    """(count, stopped, over_budget): the count of match(), or None when
+    the search tried more than about max_steps candidates (over_budget).
+    stopped: more than `limit` embeddings were seen (count partial)."""
+    off, adj, labels = _csr(off, adj, labels)
+    ql = np.ascontiguousarray(qlabels, dtype=np.int32)
+    qa = np.ascontiguousarray(qadjm, dtype=np.uint32)
+    stopped, over = C.c_int(0), C.c_int(0)
+    cnt = lib().oracle_match_budget(len(ql), _p(ql, _i32p), _p(qa), len(labels), _p(off), _p(adj), _p(labels, _i32p),
+                                    threads, limit, max_steps, C.byref(stopped), C.byref(over))
+    return (None if over.value else int(cnt)), bool(stopped.value), bool(over.value)
 
 
 def match_digest(qlabels, qadjm, off, adj, labels, threads: int = 1) -> Tuple[int, Tuple[int, int]]:

@@ -35,6 +35,9 @@
  *   oracle_check_rows   validity of given rows (labels, injectivity, every
  *                       query edge present) and their row hashes; pinned by
  *                       planted mistakes in known embeddings.
+ *   oracle_match_budget oracle_match_limit under a step budget (candidates
+ *                       tried), for checks that skip the queries whose search
+ *                       is too large; a finished search is oracle_match's.
  *   oracle_triangles    number of triangles by degree-ordered orientation
  *                       (each triangle once; the triangle query's embeddings
  *                       are 6 times it); pinned by oracle_match and tr(A^3).
@@ -276,6 +279,10 @@ typedef struct {
   uint64_t seen;
   int stop;
   int digest;          /* fold every embedding into dig[] */
+  /* step budget: stop once this many candidates were tried (0 = none) */
+  uint64_t max_steps;
+  uint64_t steps;
+  int over_budget;
 } Match;
 
 static int has_edge(const Match* M, uint32_t a, uint32_t b) {
@@ -294,6 +301,7 @@ typedef struct {
   uint32_t buf[64 * OMAXQ];
   uint32_t nbuf;
   uint64_t dig[2];
+  uint64_t steps;
 } Worker;
 
 /* Row hash of the set digest: splitmix64's finaliser (public domain) chained
@@ -343,7 +351,7 @@ static void emit(Worker* W) {
 static void extend(Worker* W, uint32_t p) {
   Match* M = W->M;
   if (p == M->k) { emit(W); return; }
-  if (M->limit && __atomic_load_n(&M->stop, __ATOMIC_RELAXED)) return;
+  if ((M->limit || M->max_steps) && __atomic_load_n(&M->stop, __ATOMIC_RELAXED)) return;
   const uint32_t u = M->order[p];
   /* iterate the neighbours of the backward neighbour with the smallest image degree */
   uint32_t pivot = OINVALID, best = OINVALID;
@@ -355,6 +363,14 @@ static void extend(Worker* W, uint32_t p) {
   uint32_t lo, hi;
   if (pivot == OINVALID) { lo = 0; hi = M->n; } else { lo = M->off[pivot]; hi = M->off[pivot + 1]; }
   for (uint32_t i = lo; i < hi; i++) {
+    if (M->max_steps && (++W->steps & 65535u) == 0) {
+      const uint64_t s = __atomic_add_fetch(&M->steps, 65536, __ATOMIC_RELAXED);
+      if (s > M->max_steps) {
+        __atomic_store_n(&M->over_budget, 1, __ATOMIC_RELAXED);
+        __atomic_store_n(&M->stop, 1, __ATOMIC_RELAXED);
+      }
+      if (__atomic_load_n(&M->stop, __ATOMIC_RELAXED)) return;
+    }
     uint32_t x = pivot == OINVALID ? i : M->adj[i];
     if (M->labels[x] != M->ql[u]) continue;
     if (M->off[x + 1] - M->off[x] < M->qdeg[u]) continue;
@@ -375,7 +391,7 @@ static void* worker_main(void* arg) {
   Match* M = W->M;
   for (;;) {
     uint32_t r = __atomic_fetch_add(&M->next_root, 1, __ATOMIC_RELAXED);
-    if (r >= M->nroots || (M->limit && __atomic_load_n(&M->stop, __ATOMIC_RELAXED))) break;
+    if (r >= M->nroots || ((M->limit || M->max_steps) && __atomic_load_n(&M->stop, __ATOMIC_RELAXED))) break;
     W->phi[0] = M->roots[r];
     extend(W, 1);
   }
@@ -391,7 +407,7 @@ static void* worker_main(void* arg) {
 static uint64_t match_run(uint32_t k, const int32_t* qlabels, const uint32_t* qadjm, uint32_t n,
                           const uint32_t* off, const uint32_t* adj, const int32_t* labels, uint32_t threads,
                           uint32_t* rows, uint64_t cap, uint64_t* written, uint64_t limit, int* stopped,
-                          uint64_t* digest) {
+                          uint64_t* digest, uint64_t max_steps, int* over_budget) {
   if (digest) digest[0] = digest[1] = 0;
   if (k == 0 || k > OMAXQ) return 0;
   Match M;
@@ -407,6 +423,7 @@ static uint64_t match_run(uint32_t k, const int32_t* qlabels, const uint32_t* qa
   M.rows = rows;
   M.cap = rows ? cap : 0;
   M.limit = limit;
+  M.max_steps = max_steps;
   for (uint32_t u = 0; u < k; u++) M.qdeg[u] = (uint32_t)__builtin_popcount(qadjm[u]);
   /* order: max degree first, then the vertex with most ordered neighbours */
   uint32_t used = 0;
@@ -448,7 +465,8 @@ static uint64_t match_run(uint32_t k, const int32_t* qlabels, const uint32_t* qa
       digest[1] += W[t].dig[1];
     }
   }
-  if (stopped) *stopped = M.stop;
+  if (stopped) *stopped = M.stop && !M.over_budget;
+  if (over_budget) *over_budget = M.over_budget;
   if (written) *written = M.written;
   pthread_mutex_destroy(&M.lock);
   free(W);
@@ -460,13 +478,24 @@ static uint64_t match_run(uint32_t k, const int32_t* qlabels, const uint32_t* qa
 uint64_t oracle_match_limit(uint32_t k, const int32_t* qlabels, const uint32_t* qadjm, uint32_t n,
                             const uint32_t* off, const uint32_t* adj, const int32_t* labels, uint32_t threads,
                             uint32_t* rows, uint64_t cap, uint64_t* written, uint64_t limit, int* stopped) {
-  return match_run(k, qlabels, qadjm, n, off, adj, labels, threads, rows, cap, written, limit, stopped, NULL);
+  return match_run(k, qlabels, qadjm, n, off, adj, labels, threads, rows, cap, written, limit, stopped, NULL, 0,
+                   NULL);
+}
+
+/* oracle_match_limit with a step budget: the search also stops once about
+ * max_steps candidates were tried, and *over_budget tells whether it did
+ * (the count is then partial and says nothing). */
+uint64_t oracle_match_budget(uint32_t k, const int32_t* qlabels, const uint32_t* qadjm, uint32_t n,
+                             const uint32_t* off, const uint32_t* adj, const int32_t* labels, uint32_t threads,
+                             uint64_t limit, uint64_t max_steps, int* stopped, int* over_budget) {
+  return match_run(k, qlabels, qadjm, n, off, adj, labels, threads, NULL, 0, NULL, limit, stopped, NULL, max_steps,
+                   over_budget);
 }
 
 uint64_t oracle_match(uint32_t k, const int32_t* qlabels, const uint32_t* qadjm, uint32_t n, const uint32_t* off,
                       const uint32_t* adj, const int32_t* labels, uint32_t threads, uint32_t* rows, uint64_t cap,
                       uint64_t* written) {
-  return match_run(k, qlabels, qadjm, n, off, adj, labels, threads, rows, cap, written, 0, NULL, NULL);
+  return match_run(k, qlabels, qadjm, n, off, adj, labels, threads, rows, cap, written, 0, NULL, NULL, 0, NULL);
 }
 
 /* Count and set digest (digest[0] = sum of row hashes x, digest[1] = sum of
@@ -474,7 +503,7 @@ uint64_t oracle_match(uint32_t k, const int32_t* qlabels, const uint32_t* qadjm,
 uint64_t oracle_match_digest(uint32_t k, const int32_t* qlabels, const uint32_t* qadjm, uint32_t n,
                              const uint32_t* off, const uint32_t* adj, const int32_t* labels, uint32_t threads,
                              uint64_t* digest) {
-  return match_run(k, qlabels, qadjm, n, off, adj, labels, threads, NULL, 0, NULL, 0, NULL, digest);
+  return match_run(k, qlabels, qadjm, n, off, adj, labels, threads, NULL, 0, NULL, 0, NULL, digest, 0, NULL);
 }
 
 /* ---------------------------------------------------------------------------

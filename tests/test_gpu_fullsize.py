@@ -12,6 +12,15 @@
       graph of the same generator and density; the first 10^8 embeddings of
       the triangle, 4-cycle, 4-clique and house are each checked row by row
       (labels, injectivity, every query edge) and for distinctness.
+  C4  Chung-Lu 1M V / 10M E, 32 Zipf(1.2) labels, the 256 walk queries of the
+      workload law counted to 1e8 under the 2e9-row work limit: the bounded
+      batch answers every query the plain loop answers, with the same count;
+      the oracle agrees on every query whose search ends within 5*10^8 tried
+      candidates (the others are skipped and the number checked printed).
+  C5  R-MAT 2^24 V / 200M E, 64 labels, the drawn 128-request stream: each
+      served result equals the same query alone.  (The oracle finishes none
+      of these requests within the step budget: its backtracking filters
+      nothing on a 16M-vertex graph; the engine's counts are pinned at C2-C4.)
 
 Each step prints what it runs before it runs it."""
 import os
@@ -131,3 +140,86 @@ def test_c3_first_1e8_embeddings_valid_and_distinct(c3, name):
             same = np.nonzero(hs == h[i])[0]
             assert len({tuple(rows[j]) for j in same}) == len(same), name
     say(f"C3 {name}: ok")
+
+
+ORACLE_STEPS = 5 * 10 ** 8
+
+
+def _alone(dg, qs, bound):
+    out = []
+    for q in qs:
+        try:
+            out.append(F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT, max_embeddings=bound).count)
+        except F.FsigError as e:
+            assert e.code == F.FSIG_ELIMIT
+            out.append(None)
+    return out
+
+
+def _oracle_agrees(name, g, qs, got, bound):
+    off, adj, lab = g.arrays()
+    checked = 0
+    for i, (q, c) in enumerate(zip(qs, got)):
+        if i % 32 == 0:
+            say(f"{name}: oracle on queries {i}..{min(i + 32, len(qs)) - 1}")
+        if c is None:
+            continue
+        want, stopped, over = O.match_budget(q.labels, q.adjm(), off, adj, lab, threads=THREADS, limit=bound,
+                                             max_steps=ORACLE_STEPS)
+        if over:
+            continue
+        assert c == (bound if stopped else min(want, bound)), (name, q.k, q.edges.tolist())
+        checked += 1
+    say(f"{name}: the oracle finished {checked} of {len(qs)} queries within {ORACLE_STEPS} steps; all agree")
+
+
+def test_c4_bounded_batch_equals_plain_and_oracle(big_ctx):
+    say("C4 graph and queries")
+    g = W.c4_graph()
+    qs = W.c4_queries(g)
+    p = W.C4_QUERIES
+    dg = big_ctx.upload(g)
+    big_ctx.set_work_limit(p["work_limit"])
+    try:
+        say("C4 plain loop")
+        plain = _alone(dg, qs, p["bound"])
+        say(f"C4 bounded batch ({sum(c is None for c in plain)} refused by the plain loop)")
+        counts, _, st = F.match_batch(dg, qs, bounds=p["bound"], status=True)
+    finally:
+        big_ctx.set_work_limit(0)
+    for q, a, c, s in zip(qs, plain, counts.tolist(), st.tolist()):
+        if a is not None:
+            assert s == 0 and c == a, (q.k, q.edges.tolist(), a, c, s)
+    say("C4 oracle under a step budget")
+    _oracle_agrees("C4", g, qs, plain, p["bound"])
+    dg.free()
+
+
+def test_c5_served_equals_alone(big_ctx):
+    say("C5 graph and request stream")
+    g = W.c5_graph()
+    qs = [q for _, q in W.c5_stream(g)]
+    p = W.C5_STREAM
+    dg = big_ctx.upload(g)
+    big_ctx.set_work_limit(p["work_limit"])
+    try:
+        say("C5 each request alone")
+        alone = _alone(dg, qs, p["bound"])
+    finally:
+        big_ctx.set_work_limit(0)
+    say("C5 the stream served (4 workers, batches of up to 64)")
+    with F.Server(dg, workers=4, hbm_budget=64 << 30, max_batch=64) as srv:
+        srv.set_work_limit(p["work_limit"])
+        tickets = [srv.submit(q, p["bound"]) for q in qs]
+        served = []
+        for t in tickets:
+            try:
+                served.append(srv.wait(t)[0])
+            except F.FsigError as e:
+                assert e.code == F.FSIG_ELIMIT
+                served.append(None)
+    for q, a, b in zip(qs, alone, served):
+        if a is not None:
+            assert b == a, (q.k, q.edges.tolist(), a, b)
+    say(f"C5: {sum(a is not None for a in alone)} of {len(qs)} requests answered alone, served equal")
+    dg.free()

@@ -8,6 +8,7 @@ import pytest
 
 import fsig_amd as F
 import oracle as O
+from fsig_amd import workloads as W
 from conftest import adjm_from_edges, csr_from_edges, load_golden
 
 
@@ -284,3 +285,19 @@ def test_triangles_equal_backtracking():
     g = F.HostGraph.from_edges(n, [(u, v) for u in range(n) for v in range(u + 1, n)], [0] * n)
     off, adj, _ = g.arrays()
     assert O.triangles(off, adj) == 84
+
+
+def test_match_budget_finished_equals_match_and_small_budget_reports():
+    """A search that finishes within its step budget counts what match()
+    counts; one that does not reports over_budget and no count."""
+    g = F.HostGraph.chung_lu(3000, 15000, 2.3, 9).set_labels("uniform", 3, seed=10)
+    off, adj, lab = g.arrays()
+    for _, q in W.walk_queries(g, 12, 3, 7, 5):
+        want, _ = O.match(q.labels, q.adjm(), off, adj, lab, threads=4)
+        got, stopped, over = O.match_budget(q.labels, q.adjm(), off, adj, lab, threads=4, max_steps=10 ** 12)
+        assert (got, stopped, over) == (want, False, False)
+    k = 5
+    q = F.Query(k, np.zeros(k, np.int32), [(a, b) for a in range(k) for b in range(a + 1, k)])
+    u = np.zeros(g.n, np.int32)
+    got, stopped, over = O.match_budget(q.labels, q.adjm(), off, adj, u, threads=4, max_steps=200_000)
+    assert over and got is None and not stopped
