@@ -103,6 +103,25 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_seg_expand(const PT* __restrict_
   }
 }
 
+// Work items [0, n) of a grid whose blocks each run `per_block` workers
+// (lane groups or wavefronts): each XCD walks its own contiguous eighth, so
+// the items one pair owns, and that pair's target bitmap, stay in one L2 at
+// a time; a grid that is no multiple of 8 strides over all items.
+struct XcdRange {
+  uint64_t lo, hi, first, stride;
+};
+__device__ __forceinline__ XcdRange xcd_range(uint64_t n, uint32_t per_block, uint32_t worker) {
+  XcdRange r{0, n, (uint64_t)blockIdx.x * per_block + worker, (uint64_t)gridDim.x * per_block};
+  if ((gridDim.x & 7u) == 0) {
+    const uint32_t xcd = blockIdx.x & 7u;
+    r.lo = n * xcd / 8;
+    r.hi = n * (xcd + 1) / 8;
+    r.first = r.lo + (uint64_t)(blockIdx.x >> 3) * per_block + worker;
+    r.stride = (uint64_t)(gridDim.x >> 3) * per_block;
+  }
+  return r;
+}
+
 // Rows of at most kHubDeg entries, G lanes each; U strides of G entries are
 // loaded before any is tested, so a group keeps U gathers in flight.
 constexpr int kRowUnroll = 4;
@@ -115,8 +134,8 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_row_count(const PT* __restrict__
   constexpr int GPB = FSIG_BLOCK / G, U = kRowUnroll;
   const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
   const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
-  const uint64_t ngroups = (uint64_t)gridDim.x * GPB;
-  for (uint64_t f = (uint64_t)xcd_block() * GPB + gw; f < rows; f += ngroups) {
+  const XcdRange xr = xcd_range(rows, GPB, gw);
+  for (uint64_t f = xr.first; f < xr.hi; f += xr.stride) {
     const PT& pd = cedge_pair(P, np, f);
     const uint32_t v = pd.acand[f - pd.flat];
     const uint32_t s = off[v], e = off[v + 1];
@@ -150,8 +169,8 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_row_fill(const PT* __restrict__ 
   const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
   const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
   const uint64_t below = (1ull << lane) - 1;
-  const uint64_t ngroups = (uint64_t)gridDim.x * GPB;
-  for (uint64_t f = (uint64_t)xcd_block() * GPB + gw; f < rows; f += ngroups) {
+  const XcdRange xr = xcd_range(rows, GPB, gw);
+  for (uint64_t f = xr.first; f < xr.hi; f += xr.stride) {
     const PT& pd = cedge_pair(P, np, f);
     const uint32_t i0 = (uint32_t)(f - pd.flat);
     const uint32_t v = pd.acand[i0];
@@ -210,8 +229,8 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_seg_cedge(const PT* __restrict__
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t below = (1ull << lane) - 1;
   const uint64_t nsegs = segbase[rows];
-  const uint64_t nwaves = (uint64_t)gridDim.x * WPB;
-  for (uint64_t s = (uint64_t)xcd_block() * WPB + (threadIdx.x >> 6); s < nsegs; s += nwaves) {
+  const XcdRange xr = xcd_range(nsegs, WPB, threadIdx.x >> 6);
+  for (uint64_t s = xr.first; s < xr.hi; s += xr.stride) {
     const uint64_t f = seg_row[s];
     const PT& pd = P[seg_pair[s]];
     const uint2 range = seg_range[s];
