@@ -203,9 +203,10 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_row_fill(const PT* __restrict__ 
   }
 }
 
-// One segment of kSegEdges = 8 x 64 entries per wavefront; segment s
-// belongs to row seg_row[s] of pair seg_pair[s], and segbase[f] (exclusive
-// scan of the segments per row) is row f's first segment.  FILL == false: the segment's entries are tested (all eight
+// One segment of kSegEdges = 8 x 64 entries per wavefront, for the
+// segments [s_lo, s_hi); segment s belongs to row seg_row[s] of pair
+// seg_pair[s], and segbase[f] (exclusive scan of the segments per row) is
+// row f's first segment.  FILL == false: the segment's entries are tested (all eight
 // loads in flight at once), the kept ones recorded as eight 64-bit words
 // segbits[8s..8s+7], their number as segcnt[s] and added to the row's count
 // cnt[f].  FILL == true: the kept entries (from segbits, no second test) are
@@ -224,13 +225,13 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_seg_cedge(const PT* __restrict__
                                                           unsigned long long* __restrict__ segbits,
                                                           uint32_t* __restrict__ cnt,
                                                           const uint64_t* __restrict__ gscan,
-                                                          uint32_t* __restrict__ lst_all) {
+                                                          uint32_t* __restrict__ lst_all, uint64_t s_lo,
+                                                          uint64_t s_hi) {
   constexpr uint32_t WPB = FSIG_BLOCK / 64;
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t below = (1ull << lane) - 1;
-  const uint64_t nsegs = segbase[rows];
-  const XcdRange xr = xcd_range(nsegs, WPB, threadIdx.x >> 6);
-  for (uint64_t s = xr.first; s < xr.hi; s += xr.stride) {
+  const XcdRange xr = xcd_range(s_hi - s_lo, WPB, threadIdx.x >> 6);
+  for (uint64_t s = s_lo + xr.first; s < s_lo + xr.hi; s += xr.stride) {
     const uint64_t f = seg_row[s];
     const PT& pd = P[seg_pair[s]];
     const uint2 range = seg_range[s];
@@ -340,24 +341,27 @@ inline uint32_t seg_grid(const fsig_ctx* c, uint64_t nsegs) {
   return (uint32_t)std::max<uint64_t>(b, 1);
 }
 
+// s_hi == ~0: every segment.
 template <class PT>
 int seg_count(fsig_ctx* c, const PT* dP, uint32_t np, uint64_t rows, const uint32_t* off, const uint32_t* adj,
-              uint32_t* cnt, const SegRows& sr) {
-  if (!sr.nsegs) return FSIG_OK;
-  k_seg_cedge<PT, false><<<seg_grid(c, sr.nsegs), FSIG_BLOCK, 0, c->stream>>>(dP, rows, sr.segbase, sr.seg_row,
-                                                                             sr.seg_pair, sr.seg_range, adj,
-                                                                             sr.segcnt, sr.segbits, cnt, nullptr, nullptr);
+              uint32_t* cnt, const SegRows& sr, uint64_t s_lo = 0, uint64_t s_hi = ~0ull) {
+  s_hi = std::min(s_hi, sr.nsegs);
+  if (s_lo >= s_hi) return FSIG_OK;
+  k_seg_cedge<PT, false><<<seg_grid(c, s_hi - s_lo), FSIG_BLOCK, 0, c->stream>>>(
+      dP, rows, sr.segbase, sr.seg_row, sr.seg_pair, sr.seg_range, adj, sr.segcnt, sr.segbits, cnt, nullptr, nullptr,
+      s_lo, s_hi);
   FSIG_HIP(c, hipGetLastError());
   return FSIG_OK;
 }
 
 template <class PT>
 int seg_fill(fsig_ctx* c, const PT* dP, uint32_t np, uint64_t rows, const uint32_t* off, const uint32_t* adj,
-             const uint64_t* gscan, uint32_t* lst_all, const SegRows& sr) {
-  if (!sr.nsegs) return FSIG_OK;
-  k_seg_cedge<PT, true><<<seg_grid(c, sr.nsegs), FSIG_BLOCK, 0, c->stream>>>(dP, rows, sr.segbase, sr.seg_row,
-                                                                            sr.seg_pair, sr.seg_range, adj,
-                                                                            sr.segcnt, sr.segbits, nullptr, gscan, lst_all);
+             const uint64_t* gscan, uint32_t* lst_all, const SegRows& sr, uint64_t s_lo = 0, uint64_t s_hi = ~0ull) {
+  s_hi = std::min(s_hi, sr.nsegs);
+  if (s_lo >= s_hi) return FSIG_OK;
+  k_seg_cedge<PT, true><<<seg_grid(c, s_hi - s_lo), FSIG_BLOCK, 0, c->stream>>>(
+      dP, rows, sr.segbase, sr.seg_row, sr.seg_pair, sr.seg_range, adj, sr.segcnt, sr.segbits, nullptr, gscan,
+      lst_all, s_lo, s_hi);
   FSIG_HIP(c, hipGetLastError());
   return FSIG_OK;
 }

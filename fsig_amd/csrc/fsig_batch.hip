@@ -235,6 +235,13 @@ __global__ void k_bscatter_pos(const uint32_t* __restrict__ cand, uint32_t n, ui
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) pos[cand[j]] = j;
 }
 
+// out[j] = src[idx[j]] for j < m.
+__global__ void k_gather_at(const uint64_t* __restrict__ src, const uint64_t* __restrict__ idx, uint32_t m,
+                            uint64_t* __restrict__ out) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) out[j] = src[idx[j]];
+}
+
 __global__ void k_bpair_base(const uint64_t* __restrict__ gscan, const BPair* __restrict__ P, uint32_t np,
                              uint64_t* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -295,13 +302,13 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_count_multi(const BSrc* _
                                                                    uint64_t total, const BPair* __restrict__ P,
                                                                    const uint32_t* __restrict__ off,
                                                                    const uint32_t* __restrict__ adj,
-                                                                   uint32_t* __restrict__ cnt) {
+                                                                   uint32_t* __restrict__ cnt, uint64_t f_lo) {
   constexpr int GPB = FSIG_BLOCK / G;
   __shared__ uint32_t acc[GPB][MAXQ];
   const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
   const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
-  const uint64_t ngroups = (uint64_t)gridDim.x * GPB;
-  for (uint64_t f = (uint64_t)xcd_block() * GPB + gw; f < total; f += ngroups) {
+  const XcdRange xr = xcd_range(total - f_lo, GPB, gw);  // source rows [f_lo, total)
+  for (uint64_t f = f_lo + xr.first; f < f_lo + xr.hi; f += xr.stride) {
     const BSrc& sc = S[find_flat(S, ns, f)];
     const uint32_t i0 = (uint32_t)(f - sc.flat);
     const uint32_t v = sc.acand[i0];
@@ -333,14 +340,14 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_fill_multi(const BSrc* __
                                                                   const uint32_t* __restrict__ off,
                                                                   const uint32_t* __restrict__ adj,
                                                                   const uint64_t* __restrict__ gscan,
-                                                                  uint32_t* __restrict__ lst_all) {
+                                                                  uint32_t* __restrict__ lst_all, uint64_t f_lo) {
   constexpr int GPB = FSIG_BLOCK / G;
   __shared__ uint64_t at[GPB][MAXQ];
   const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
   const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
   const uint64_t below = (1ull << lane) - 1;
-  const uint64_t ngroups = (uint64_t)gridDim.x * GPB;
-  for (uint64_t f = (uint64_t)xcd_block() * GPB + gw; f < total; f += ngroups) {
+  const XcdRange xr = xcd_range(total - f_lo, GPB, gw);  // source rows [f_lo, total)
+  for (uint64_t f = f_lo + xr.first; f < f_lo + xr.hi; f += xr.stride) {
     const BSrc& sc = S[find_flat(S, ns, f)];
     const uint32_t i0 = (uint32_t)(f - sc.flat);
     const uint32_t v = sc.acand[i0];
@@ -1462,8 +1469,10 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
   // ---- orders and the class pairs each query's join needs
   std::vector<BPair> pairs;
   std::vector<std::vector<int>> pair_of(nq);  // [i][a*MAXQ+b] -> pair index
+  std::vector<uint64_t> pair_row_lo(nq + 1, 0);  // query i's pair rows start here
   uint64_t pflat = 0;
   for (uint32_t i = 0; i < nq; i++) {
+    pair_row_lo[i] = pflat;
     if (!active[i]) continue;
     QueryPlan& P = plans[i];
     plan_order_mmatch(P, &size[cls_base[i]]);
@@ -1485,6 +1494,7 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
       }
     }
   }
+  pair_row_lo[nq] = pflat;
   const uint32_t np = (uint32_t)pairs.size();
   std::vector<uint64_t> pbase_h(np), maxlen_h(np), ptotal_h(np);
   uint32_t* lst_all = nullptr;
@@ -1509,49 +1519,88 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
         pd.bpos = it->second;
       }
     }
-    // candidate bitmaps of every target class, n bits each, while they take
-    // at most a quarter of the arena's free space: a bitmap stays in L2
-    // while one pair's entries are tested, a query's 4n-byte class mask
-    // does not on large graphs
+    // Candidate bitmaps of the target classes, n bits each: a bitmap stays
+    // in L2 while one pair's entries are tested, a query's 4n-byte class
+    // mask does not on large graphs.  All of a batch's bitmaps are built at
+    // once while they fit the last-level cache (kBitmapCache); beyond that
+    // the build runs query by query, each query's bitmaps rebuilt in one
+    // reused region right before its rows and segments are swept (bitmaps
+    // built long before their sweep are read back from HBM).  No room in the
+    // arena (a quarter of what is free): the class masks.
     const uint64_t nwords = (n + 31ull) / 32;
     std::vector<uint32_t*> tbits(std::max<uint32_t>(ncls, 1), nullptr);
-    std::vector<uint32_t> tgt;
+    std::vector<std::vector<uint32_t>> qtgt(nq);
+    size_t ntgt = 0, max_t = 0;
     {
       std::vector<uint8_t> seen(std::max<uint32_t>(ncls, 1), 0);
-      for (uint32_t i = 0; i < nq; i++)
+      for (uint32_t i = 0; i < nq; i++) {
         for (uint32_t x = 0; x < MAXQ * MAXQ && active[i]; x++)
           if (pair_of[i][x] >= 0) {
             const uint32_t gb = cls_base[i] + x % MAXQ;
-            if (!seen[gb]) { seen[gb] = 1; tgt.push_back(gb); }
+            if (!seen[gb]) { seen[gb] = 1; qtgt[i].push_back(gb); }
           }
+        ntgt += qtgt[i].size();
+        max_t = std::max(max_t, qtgt[i].size());
+      }
     }
-    const bool use_bits = !tgt.empty() && 4ull * nwords * tgt.size() <= A.avail() / 4;
+    constexpr uint64_t kBitmapCache = 192ull << 20;
+    const bool per_query = 4ull * nwords * ntgt > kBitmapCache;
+    const uint64_t region_words = nwords * (per_query ? max_t : ntgt);
+    const bool use_bits = ntgt && 4ull * region_words <= A.avail() / 4;
+    if (clk.on)
+      fprintf(stderr, "[fsig batch] %zu target classes: %s\n", ntgt,
+              !use_bits ? "class masks (no room for bitmaps)"
+                        : per_query ? "candidate bitmaps, query by query" : "candidate bitmaps, all at once");
+    uint32_t* region = nullptr;
+    // chunks of the build: every query alone (per_query) or the whole batch;
+    // chunk j: bitmap table entries [bt_lo[j], bt_lo[j+1]) (flats local to
+    // the chunk), source rows [src_lo[j], src_hi[j]), pair rows [row_lo[j],
+    // row_hi[j])
+    std::vector<BBits> bt;
+    std::vector<uint32_t> chunk_q;
+    std::vector<size_t> bt_lo;
+    std::vector<uint64_t> bt_total, chunk_words;
     if (use_bits) {
-      std::vector<BBits> bt;
+      region = (uint32_t*)A.alloc(4ull * region_words);
+      if (!region) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted for candidate bitmaps");
+      size_t slot = 0;
       uint64_t bflat = 0;
-      uint32_t* base = (uint32_t*)A.alloc(4ull * nwords * tgt.size());
-      if (!base) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted for candidate bitmaps");
-      for (size_t j = 0; j < tgt.size(); j++) {
-        tbits[tgt[j]] = base + j * nwords;
-        if (size[tgt[j]]) {
-          bt.push_back(BBits{cand[tgt[j]], tbits[tgt[j]], size[tgt[j]], 0, bflat});
-          bflat += size[tgt[j]];
+      for (uint32_t i = 0; i < nq; i++) {
+        if (qtgt[i].empty()) continue;
+        if (per_query || chunk_q.empty()) {  // a chunk starts
+          chunk_q.push_back(i);
+          bt_lo.push_back(bt.size());
+          bflat = 0;
+          slot = 0;
+        }
+        for (uint32_t gb : qtgt[i]) {
+          tbits[gb] = region + slot * nwords;
+          slot++;
+          if (size[gb]) {
+            bt.push_back(BBits{cand[gb], tbits[gb], size[gb], 0, bflat});
+            bflat += size[gb];
+          }
+        }
+        if (per_query) {
+          bt_total.push_back(bflat);
+          chunk_words.push_back(slot * nwords);
         }
       }
-      k_fill_u32<<<bgrid(c, nwords * tgt.size(), FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(base, nwords * tgt.size(), 0);
-      const BBits* dbt = upload_table(c, bt);
-      if (!dbt) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted");
-      if (bflat)
-        k_bset_bits<<<bgrid(c, bflat, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(dbt, (uint32_t)bt.size(), bflat);
-      FSIG_HIP(c, hipGetLastError());
+      if (!per_query && !chunk_q.empty()) {
+        bt_total.push_back(bflat);
+        chunk_words.push_back(slot * nwords);
+      }
+      bt_lo.push_back(bt.size());
       for (uint32_t i = 0; i < nq; i++)
         for (uint32_t x = 0; x < MAXQ * MAXQ && active[i]; x++)
           if (pair_of[i][x] >= 0) pairs[pair_of[i][x]].bits = tbits[cls_base[i] + x % MAXQ];
     }
     // sources: every (query, class a) with its targets, one sweep each
     std::vector<BSrc> srcs;
+    std::vector<uint64_t> qsrc_lo(nq + 1, 0), qrow_lo(nq + 1, 0);
     uint64_t sflat = 0;
     for (uint32_t i = 0; i < nq; i++) {
+      qsrc_lo[i] = sflat;
       if (!active[i]) continue;
       for (uint32_t a = 0; a < plans[i].nclass; a++) {
         BSrc sc;
@@ -1573,38 +1622,82 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
         if (size[ga]) srcs.push_back(sc);
       }
     }
+    qsrc_lo[nq] = sflat;
+    for (uint32_t i = 0; i <= nq; i++) qrow_lo[i] = pair_row_lo[i];
     const BPair* dP = upload_table(c, pairs);
     const BSrc* dS = upload_table(c, srcs);
+    const BBits* dbt = upload_table(c, bt);
     const uint32_t nsrc = (uint32_t)srcs.size();
     uint32_t* cnt = (uint32_t*)A.alloc(4ull * std::max<uint64_t>(pflat, 1));
     uint64_t* gscan = (uint64_t*)A.alloc(8ull * (pflat + 1));
     unsigned int* dmax = (unsigned int*)A.alloc(4ull * np);
     uint64_t* pb = (uint64_t*)A.alloc(8ull * np);
-    if (!dP || !dS || !cnt || !gscan || !dmax || !pb) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted");
-    if (sflat) {
-      switch (RG) {
-        case 8: k_bcedge_count_multi<8><<<bgrid(c, sflat, FSIG_BLOCK / 8), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, cnt); break;
-        case 16: k_bcedge_count_multi<16><<<bgrid(c, sflat, FSIG_BLOCK / 16), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, cnt); break;
-        default: k_bcedge_count_multi<64><<<bgrid(c, sflat, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, cnt); break;
-      }
-    }
+    if (!dP || !dS || (!bt.empty() && !dbt) || !cnt || !gscan || !dmax || !pb)
+      return set_err(c, FSIG_ENOMEM, "batch: arena exhausted");
     SegRows segs;  // hub rows of every pair, one wavefront per segment
     if ((rc = seg_prepare(c, dP, np, pflat, g->d_off, cnt, segs))) return rc;
-    if ((rc = seg_count(c, dP, np, pflat, g->d_off, g->d_adj, cnt, segs))) return rc;
+    // the chunks' source, pair-row and segment ranges
+    const size_t nch = use_bits ? chunk_q.size() : 1;
+    std::vector<uint64_t> ch_src(nch + 1), ch_row(nch + 1), ch_seg(nch + 1, 0);
+    for (size_t j = 0; j < nch; j++) {
+      const uint32_t i = use_bits ? chunk_q[j] : 0;
+      ch_src[j] = use_bits ? qsrc_lo[i] : 0;
+      ch_row[j] = use_bits ? qrow_lo[i] : 0;
+    }
+    ch_src[nch] = sflat;
+    ch_row[nch] = pflat;
+    if (!per_query || !use_bits) {
+      ch_src[0] = 0;
+      ch_row[0] = 0;
+    }
+    if (segs.nsegs) {
+      uint64_t* didx = (uint64_t*)upload_table(c, ch_row);
+      uint64_t* dseg = (uint64_t*)A.alloc(8ull * (nch + 1));
+      if (!didx || !dseg) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted");
+      k_gather_at<<<(uint32_t)((nch + 1 + 255) / 256), 256, 0, c->stream>>>(segs.segbase, didx, (uint32_t)(nch + 1), dseg);
+      FSIG_HIP(c, hipGetLastError());
+      if ((rc = read_u64(c, dseg, ch_seg.data(), nch + 1))) return rc;
+    }
+    auto build_bits = [&](size_t j) -> int {
+      if (!use_bits) return FSIG_OK;
+      k_fill_u32<<<bgrid(c, chunk_words[j], FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(region, chunk_words[j], 0);
+      const size_t nt = bt_lo[j + 1] - bt_lo[j];
+      if (nt && bt_total[j])
+        k_bset_bits<<<bgrid(c, bt_total[j], FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(dbt + bt_lo[j], (uint32_t)nt,
+                                                                                     bt_total[j]);
+      FSIG_HIP(c, hipGetLastError());
+      return FSIG_OK;
+    };
+    for (size_t j = 0; j < nch; j++) {
+      if ((rc = build_bits(j))) return rc;
+      const uint64_t lo = ch_src[j], hi = ch_src[j + 1];
+      if (hi > lo) {
+        switch (RG) {
+          case 8: k_bcedge_count_multi<8><<<bgrid(c, hi - lo, FSIG_BLOCK / 8), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, hi, dP, g->d_off, g->d_adj, cnt, lo); break;
+          case 16: k_bcedge_count_multi<16><<<bgrid(c, hi - lo, FSIG_BLOCK / 16), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, hi, dP, g->d_off, g->d_adj, cnt, lo); break;
+          default: k_bcedge_count_multi<64><<<bgrid(c, hi - lo, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, hi, dP, g->d_off, g->d_adj, cnt, lo); break;
+        }
+      }
+      if ((rc = seg_count(c, dP, np, pflat, g->d_off, g->d_adj, cnt, segs, ch_seg[j], ch_seg[j + 1]))) return rc;
+    }
     FSIG_HIP(c, hipGetLastError());
     if ((rc = scan_u32_to_u64(c, cnt, gscan, pflat))) return rc;
     uint64_t total_edges;
     if ((rc = read_u64(c, gscan + pflat, &total_edges, 1))) return rc;
     lst_all = (uint32_t*)A.alloc(4ull * std::max<uint64_t>(total_edges, 1));
     if (!lst_all) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted for candidate lists");
-    if (sflat) {
-      switch (RG) {
-        case 8: k_bcedge_fill_multi<8><<<bgrid(c, sflat, FSIG_BLOCK / 8), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, gscan, lst_all); break;
-        case 16: k_bcedge_fill_multi<16><<<bgrid(c, sflat, FSIG_BLOCK / 16), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, gscan, lst_all); break;
-        default: k_bcedge_fill_multi<64><<<bgrid(c, sflat, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, sflat, dP, g->d_off, g->d_adj, gscan, lst_all); break;
+    for (size_t j = 0; j < nch; j++) {
+      if (nch > 1 && (rc = build_bits(j))) return rc;  // one chunk: its bitmaps are still built
+      const uint64_t lo = ch_src[j], hi = ch_src[j + 1];
+      if (hi > lo) {
+        switch (RG) {
+          case 8: k_bcedge_fill_multi<8><<<bgrid(c, hi - lo, FSIG_BLOCK / 8), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, hi, dP, g->d_off, g->d_adj, gscan, lst_all, lo); break;
+          case 16: k_bcedge_fill_multi<16><<<bgrid(c, hi - lo, FSIG_BLOCK / 16), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, hi, dP, g->d_off, g->d_adj, gscan, lst_all, lo); break;
+          default: k_bcedge_fill_multi<64><<<bgrid(c, hi - lo, FSIG_BLOCK / 64), FSIG_BLOCK, 0, c->stream>>>(dS, nsrc, hi, dP, g->d_off, g->d_adj, gscan, lst_all, lo); break;
+        }
       }
+      if ((rc = seg_fill(c, dP, np, pflat, g->d_off, g->d_adj, gscan, lst_all, segs, ch_seg[j], ch_seg[j + 1]))) return rc;
     }
-    if ((rc = seg_fill(c, dP, np, pflat, g->d_off, g->d_adj, gscan, lst_all, segs))) return rc;
     k_bpair_maxlen<<<np, FSIG_BLOCK, 0, c->stream>>>(dP, gscan, dmax);
     k_bpair_base<<<(np + 255) / 256, 256, 0, c->stream>>>(gscan, dP, np, pb);
     FSIG_HIP(c, hipGetLastError());

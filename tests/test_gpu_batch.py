@@ -175,6 +175,25 @@ def test_bounded_batch_equals_bounded_plain_and_oracle(gpu_ctx, beam_log2):
             assert c == b  # the oracle saw more than b embeddings
 
 
+def test_batch_bitmaps_built_query_by_query(gpu_ctx):
+    """On a graph whose batch of candidate bitmaps outgrows the last-level
+    cache (4M vertices: 512 KB per target class, about 900 classes here),
+    the candidate-edge build runs query by query with each query's bitmaps
+    rebuilt in one region: counts equal fsig_match's, bounded and exact."""
+    g = F.HostGraph.rmat(22, 12_000_000, seed=41).set_labels("uniform", 16, seed=42)
+    dg = gpu_ctx.upload(g)
+    qs = [q for _, q in W.walk_queries(g, 128, 4, 10, 43)]
+    bound = 10 ** 6
+    plain = [F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT, max_embeddings=bound).count for q in qs]
+    counts, _, st = F.match_batch(dg, qs, bounds=bound, status=True)
+    assert not st.any()
+    assert counts.tolist() == plain
+    small = [i for i, c in enumerate(plain) if c < bound]  # exact counts below the bound
+    counts2, _ = F.match_batch(dg, [qs[i] for i in small])
+    assert counts2.tolist() == [plain[i] for i in small]
+    dg.free()
+
+
 def test_bounded_batch_work_limit_refuses_per_query(gpu_ctx):
     """A bounded query over the context's work limit is refused alone
     (FSIG_ELIMIT in its status); the others complete."""
