@@ -332,7 +332,7 @@ def run_c5(args, steps, warmup):
     plain_s = time.perf_counter() - start
     ctx.set_work_limit(0)
     log(f"C5 plain: {plain_s * 1e3:.0f} ms, p50 {np.percentile(plain_lat, 50):.1f} ms")
-    srv = F.Server(dg, workers=args.workers, hbm_budget=128 << 30, max_batch=64)
+    srv = F.Server(dg, workers=args.workers, hbm_budget=128 << 30, max_batch=args.max_batch)
     srv.set_work_limit(WL)
 
     def serve_once():
@@ -388,7 +388,7 @@ def run_c5(args, steps, warmup):
            "at_bound": sum(1 for c in plain if c == B), "bound": B, "work_limit": WL,
            "draws_without_walk": info["no_walk"],
            "k_histogram": {str(k): int(sum(1 for q in qs if q.k == k)) for k in sorted({q.k for q in qs})},
-           "workers": args.workers, "hbm_budget_GiB": 128, "max_batch": 64,
+           "workers": args.workers, "hbm_budget_GiB": 128, "max_batch": args.max_batch,
            "server": {k: (round(v - st0[k], 1) if isinstance(v, float) else v - st0[k])
                       for k, v in st.items() if k not in ("max_concurrent", "max_batch_seen", "wall_ms")}}
     dg.free()
@@ -410,7 +410,8 @@ def main():
     ap.add_argument("--workload", default="all", choices=["all", "c2", "c3", "c4", "c5", "ingest"])
     ap.add_argument("--qps", type=float, default=0.0, help="c5: arrival rate (0: the workload's)")
     ap.add_argument("--requests", type=int, default=0, help="c5: stream length (0: the workload's)")
-    ap.add_argument("--workers", type=int, default=4, help="c5: server worker contexts")
+    ap.add_argument("--workers", type=int, default=8, help="c5: server worker contexts")
+    ap.add_argument("--max-batch", type=int, default=32, help="c5: most requests a worker batches")
     ap.add_argument("--c4-steps", type=int, default=5, help="all: timed batches of C4")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--ingest-c5", action="store_true",

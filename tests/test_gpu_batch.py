@@ -181,7 +181,8 @@ def test_batch_bitmaps_built_query_by_query(gpu_ctx):
     the candidate-edge build runs query by query with each query's bitmaps
     rebuilt in one region: counts equal fsig_match's, bounded and exact."""
     g = F.HostGraph.rmat(22, 12_000_000, seed=41).set_labels("uniform", 16, seed=42)
-    dg = gpu_ctx.upload(g)
+    ctx = F.Context(0, 32 << 30)  # 128 class masks of 16 MB alone outgrow the shared test context
+    dg = ctx.upload(g)
     qs = [q for _, q in W.walk_queries(g, 128, 4, 10, 43)]
     bound = 10 ** 6
     plain = [F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT, max_embeddings=bound).count for q in qs]
@@ -192,6 +193,7 @@ def test_batch_bitmaps_built_query_by_query(gpu_ctx):
     counts2, _ = F.match_batch(dg, [qs[i] for i in small])
     assert counts2.tolist() == [plain[i] for i in small]
     dg.free()
+    ctx.close()
 
 
 def test_bounded_batch_work_limit_refuses_per_query(gpu_ctx):
