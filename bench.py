@@ -73,13 +73,28 @@ def log(*a):
 
 
 def cpu_model():
-    """The host CPU's model name (lscpu, else /proc/cpuinfo; None where
-    neither answers)."""
+    """The host CPU as lscpu describes it: model name, then family / model /
+    stepping, sockets x cores x threads and the hypervisor (a virtual
+    machine may report a shortened model name; family and model still
+    identify the part); /proc/cpuinfo's model name where lscpu does not
+    answer; None where neither does."""
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        f = {}
         for line in out.splitlines():
-            if line.startswith("Model name:"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = line.split(":", 1)
+                f[k.strip()] = v.strip()
+        if "Model name" in f:
+            parts = [f["Model name"]]
+            if "CPU family" in f:
+                parts.append(f"family {f['CPU family']} model {f.get('Model', '?')} stepping {f.get('Stepping', '?')}")
+            if "Socket(s)" in f:
+                parts.append(f"{f['Socket(s)']} socket(s) x {f.get('Core(s) per socket', '?')} cores x "
+                             f"{f.get('Thread(s) per core', '?')} threads, {f.get('CPU(s)', '?')} CPUs")
+            if "Hypervisor vendor" in f:
+                parts.append(f"{f['Hypervisor vendor']} guest")
+            return "; ".join(parts)
     except (OSError, subprocess.SubprocessError):
         pass
     try:
