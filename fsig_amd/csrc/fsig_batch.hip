@@ -1812,7 +1812,6 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
   clk.mark("level tables + tasks");
   // ---- join engines
   // Lockstep batched BFS over a set of live queries; counts land in dcounts.
-  const size_t join_mark = A.mark();
   auto fill_level = [&](uint32_t i, uint32_t p, JoinLevel& L, CandEdgeSrc& S) {
     const QueryPlan& P = plans[i];
     const uint32_t w = P.order[p];
@@ -2229,7 +2228,6 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
     rc = persistent(best, all_q, true, ms, family_counters(c, FSIG_K_BATCH));
   }
   if (rc) return rc;
-  (void)join_mark;
   std::vector<uint64_t> hc(nq);
   for (uint32_t i = 0; i < nq; i += kPinnedWords)
     if ((rc = read_u64(c, dcounts + i, hc.data() + i, std::min<size_t>(kPinnedWords, nq - i)))) return rc;

@@ -121,10 +121,11 @@ def test_batch_lanes_per_query(gpu_ctx):
 
 
 def test_batch_hub_candidates(gpu_ctx):
-    # R-MAT hubs (max degree 3201 > 2048): their candidate-edge lists come
-    # from the per-hub workgroup sweep; counts must equal the plain path's
+    # R-MAT hubs (max degree 3201, far above the 512-entry row limit): their
+    # candidate-edge lists come from segments swept one wavefront each;
+    # counts must equal the plain path's
     g = F.HostGraph.rmat(13, 200000, seed=21).set_labels("uniform", 3, seed=22)
-    assert g.max_degree > 2048
+    assert g.max_degree > 4 * 512
     dg = gpu_ctx.upload(g)
     qs, ref = zip(*_bounded(dg, [q for _, q in W.walk_queries(g, 24, 3, 6, 23)], 20_000_000))
     off, adj, lab = g.arrays()
@@ -148,8 +149,8 @@ def test_bounded_batch_equals_bounded_plain_and_oracle(gpu_ctx, beam_log2):
     fsig_match's bounded count and to the oracle (stopped searches prove
     the count exceeds the bound).  beam_log2 = 4 keeps a 16-child beam of
     every bounded query in the lockstep join, so most of them end below their
-    bound and are counted again by the depth-first driver; exact queries
-    (bound 0) are never beamed."""
+    bound and have the subtrees of their cut rows counted by the depth-first
+    driver; exact queries (bound 0) are never beamed."""
     g = F.HostGraph.chung_lu(20000, 120000, 2.3, 97).set_labels("zipf", 6, 1.2, 98)
     off, adj, lab = g.arrays()
     dg = gpu_ctx.upload(g)

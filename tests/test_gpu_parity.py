@@ -345,12 +345,13 @@ def test_lanes_per_row_do_not_change_results(gpu_ctx, lanes):
 
 
 def test_hub_candidates_unlabeled_closed_forms(gpu_ctx):
-    # R-MAT with hubs above the 2048-neighbour threshold: candidate edges of
-    # the hubs come from the per-hub workgroup sweep.  Monomorphism counts
-    # against closed forms: triangle tr(A^3), 4-cycle tr(A^4) - 2 sum d^2 + 2m
+    # R-MAT with hubs far above the 512-entry row limit: candidate edges of
+    # the hubs come from segments swept one wavefront each.  Monomorphism
+    # counts against closed forms: triangle tr(A^3), 4-cycle tr(A^4) - 2 sum
+    # d^2 + 2m
     import scipy.sparse as sp
     g = F.HostGraph.rmat(12, 150000, seed=27)
-    assert g.max_degree > 2048
+    assert g.max_degree > 4 * 512
     off, adj, _ = g.arrays()
     A = sp.csr_matrix((np.ones(len(adj)), adj.astype(np.int64), off.astype(np.int64)), shape=(g.n, g.n))
     A2 = A @ A
