@@ -1430,7 +1430,7 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
   kill_empty();
   const double avg_deg = n ? (double)g->nnz / n : 0;
   const uint32_t RG = avg_deg <= 12 ? 8 : (avg_deg <= 40 ? 16 : 64);
-  const uint32_t bpasses = refine_passes(c, true);
+  const uint32_t bpasses = refine_passes(c);
   for (uint32_t pass = 0; any_target && pass < bpasses; pass++) {
     KTimer tr(c, FSIG_K_REFINE);
     std::vector<BClass> cl;

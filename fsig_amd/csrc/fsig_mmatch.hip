@@ -338,7 +338,7 @@ int run_mmatch(fsig_ctx* c, const fsig_graph* g, QueryPlan& P, int mode, uint64_
   const uint32_t RG = refine_lanes(g);
   bool empty = false;
   for (uint32_t a = 0; a < nc; a++) empty |= size[a] == 0;
-  for (uint32_t pass = 0; any_target && !empty && pass < refine_passes(c, false); pass++) {
+  for (uint32_t pass = 0; any_target && !empty && pass < refine_passes(c); pass++) {
     KTimer tr(c, FSIG_K_REFINE);
     uint64_t flat = 0, deg_sum = 0;
     for (uint32_t a = 0; a < nc; a++) {

@@ -202,11 +202,11 @@ struct KTimer {
 // the context statistics; the stream must be idle.
 int resolve_timings(fsig_ctx* c);
 inline unsigned long long* family_counters(fsig_ctx* c, int fam) { return c->d_counters + 16 + 4 * fam; }
-// Refinement passes: the tuned cap, else 4 for one query and 1 for a batch
-// (its neighbour-label filter leaves little for later passes to remove).
-inline uint32_t refine_passes(const fsig_ctx* c, bool batch) {
-  return c->tune.refine_passes ? c->tune.refine_passes : (batch ? 1u : 4u);
-}
+// Refinement passes: the tuned cap, else 2 (one query or a batch).  After
+// the neighbour-label filter the second pass still pays for itself on large
+// graphs (smaller candidate edges and joins); later ones remove a few
+// candidates at the price of a full pass.
+inline uint32_t refine_passes(const fsig_ctx* c) { return c->tune.refine_passes ? c->tune.refine_passes : 2u; }
 
 // Device-wide exclusive scan of n u32 counts into out[0..n] (u64, out[n] is
 // the total).  Temporary storage comes from the arena.

@@ -1077,9 +1077,15 @@ class JoinDriver {
     unsigned long long* dwin = split ? (unsigned long long*)c_->arena.alloc(16) : nullptr;
     if (split && !dwin) return set_err(c_, FSIG_ENOMEM, "join: arena exhausted for windows");
     for (uint64_t lo = 0; lo < T && remaining() > 0;) {
-      const uint64_t hi = std::min(T, lo + wsize);
+      uint64_t hi = std::min(T, lo + wsize);
       if (grow) {
-        if (limit && !last && rows_ + (hi - lo) > limit) return set_err(c_, FSIG_ELIMIT, "join: work limit exceeded");
+        // the limit is exact: a window is clipped to the rows the limit
+        // leaves (the bound may be reached inside it), and the match stops
+        // only when it needs a row beyond the limit
+        if (limit && !last) {
+          if (rows_ >= limit) return set_err(c_, FSIG_ELIMIT, "join: work limit exceeded");
+          hi = std::min(hi, lo + (limit - rows_));
+        }
         rows_ += last ? 0 : hi - lo;
         window_[p] = std::min(window_[p] * 2, kGrowMax);
         wsize = std::min(cap, window_[p]);

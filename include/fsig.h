@@ -230,8 +230,8 @@ uint64_t fsig_last_peak_bytes(const fsig_ctx* ctx);
  * staged in LDS (lds_tile words per group); 3 (any lanes_per_row): lockstep
  * join with each query's lanes from its mean candidate-list length, no
  * autotuning.  For fsig_match, lds_tile caps the LDS tile of the fused tail
- * (words per lane group).  refine_passes: cap on refinement passes (0: 4
- * for fsig_match, 1 for a batch). */
+ * (words per lane group).  refine_passes: cap on refinement passes (0:
+ * 2). */
 typedef struct fsig_tune {
   uint32_t lanes_per_row;
   uint32_t strategy;

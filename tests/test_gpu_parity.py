@@ -97,14 +97,15 @@ def _deg(q):
     return [int(bin(int(m)).count("1")) for m in q.adjm()]
 
 
-def test_candidate_sets_equal_oracle_filter_and_refinement(gpu_ctx):
+@pytest.mark.parametrize("passes", [1, 2, 4])
+def test_candidate_sets_equal_oracle_filter_and_refinement(gpu_ctx, passes):
     """Element by element: the label/degree filter's sets equal oracle_ldf and
-    the refined sets equal oracle_refine (the reference's filters only)."""
+    the refined sets equal oracle_refine after the same number of passes (the
+    reference's filters only)."""
     g = F.HostGraph.chung_lu(8000, 40000, 2.5, 31).set_labels("uniform", 4, seed=32)
     off, adj, lab = g.arrays()
     dg = gpu_ctx.upload(g)
-    passes = gpu_ctx.get_tune()["refine_passes"] or 4
-    gpu_ctx.set_tune(nlf_filter=1)
+    gpu_ctx.set_tune(refine_passes=passes, nlf_filter=1)
     gpu_ctx.keep_candidates(True)
     try:
         shrunk = False
