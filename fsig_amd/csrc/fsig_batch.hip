@@ -280,17 +280,20 @@ __device__ __forceinline__ uint32_t target_bits(const BSrc& sc, uint32_t w, uint
 struct BBits {
   const uint32_t* cand;
   uint32_t* bits;
+  uint32_t* pos;  // vertex -> candidate index map (n words) or null
   uint32_t size;
   uint32_t pad;
   uint64_t flat;
 };
 
-__global__ void k_bset_bits(const BBits* __restrict__ T, uint32_t nt, uint64_t total) {
+// Bits of every class's candidates; with_pos: also pos[C[j]] = j.
+__global__ void k_bset_bits(const BBits* __restrict__ T, uint32_t nt, uint64_t total, bool with_pos) {
   for (uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; f < total;
        f += (uint64_t)gridDim.x * blockDim.x) {
     const BBits& t = T[find_flat(T, nt, f)];
     const uint32_t v = t.cand[f - t.flat];
     atomicOr(&t.bits[v >> 5], 1u << (v & 31u));
+    if (with_pos && t.pos) t.pos[v] = (uint32_t)(f - t.flat);
   }
 }
 
@@ -1500,25 +1503,6 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
   uint32_t* lst_all = nullptr;
   if (np) {
     KTimer tc(c, FSIG_K_CEDGE);
-    // vertex -> candidate index maps of the target classes, n words each,
-    // while the arena holds a quarter of its free space for them
-    {
-      std::map<const uint32_t*, uint32_t*> made;
-      const uint64_t budget = A.avail() / 4;
-      uint64_t used = 0;
-      for (auto& pd : pairs) {
-        auto it = made.find(pd.bcand);
-        if (it == made.end()) {
-          uint32_t* m = nullptr;
-          if (used + 4ull * n <= budget && (m = (uint32_t*)A.alloc(4ull * n))) {
-            used += 4ull * n;
-            k_bscatter_pos<<<bgrid(c, pd.bsize, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(pd.bcand, pd.bsize, m);
-          }
-          it = made.emplace(pd.bcand, m).first;
-        }
-        pd.bpos = it->second;
-      }
-    }
     // Candidate bitmaps of the target classes, n bits each: a bitmap stays
     // in L2 while one pair's entries are tested, a query's 4n-byte class
     // mask does not on large graphs.  All of a batch's bitmaps are built at
@@ -1552,6 +1536,15 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
               !use_bits ? "class masks (no room for bitmaps)"
                         : per_query ? "candidate bitmaps, query by query" : "candidate bitmaps, all at once");
     uint32_t* region = nullptr;
+    // Vertex -> candidate index maps of the target classes (n words each),
+    // read by the fills for the entries they keep.  Query by query: one
+    // reused region holds a query's maps, scattered with its bitmaps right
+    // before its fill, when the largest query's fit a quarter of the free
+    // arena.  Otherwise every target class has its own map while a quarter
+    // of the free arena holds them, and the classes beyond search their
+    // candidate list instead.
+    uint32_t* posregion = nullptr;
+    std::vector<uint32_t*> tpos(std::max<uint32_t>(ncls, 1), nullptr);
     // chunks of the build: every query alone (per_query) or the whole batch;
     // chunk j: bitmap table entries [bt_lo[j], bt_lo[j+1]) (flats local to
     // the chunk), source rows [src_lo[j], src_hi[j]), pair rows [row_lo[j],
@@ -1563,6 +1556,7 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
     if (use_bits) {
       region = (uint32_t*)A.alloc(4ull * region_words);
       if (!region) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted for candidate bitmaps");
+      if (per_query && 4ull * n * max_t <= A.avail() / 4) posregion = (uint32_t*)A.alloc(4ull * n * max_t);
       size_t slot = 0;
       uint64_t bflat = 0;
       for (uint32_t i = 0; i < nq; i++) {
@@ -1575,9 +1569,10 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
         }
         for (uint32_t gb : qtgt[i]) {
           tbits[gb] = region + slot * nwords;
+          if (posregion) tpos[gb] = posregion + slot * (uint64_t)n;
           slot++;
           if (size[gb]) {
-            bt.push_back(BBits{cand[gb], tbits[gb], size[gb], 0, bflat});
+            bt.push_back(BBits{cand[gb], tbits[gb], tpos[gb], size[gb], 0, bflat});
             bflat += size[gb];
           }
         }
@@ -1593,7 +1588,27 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
       bt_lo.push_back(bt.size());
       for (uint32_t i = 0; i < nq; i++)
         for (uint32_t x = 0; x < MAXQ * MAXQ && active[i]; x++)
-          if (pair_of[i][x] >= 0) pairs[pair_of[i][x]].bits = tbits[cls_base[i] + x % MAXQ];
+          if (pair_of[i][x] >= 0) {
+            pairs[pair_of[i][x]].bits = tbits[cls_base[i] + x % MAXQ];
+            pairs[pair_of[i][x]].bpos = tpos[cls_base[i] + x % MAXQ];
+          }
+    }
+    if (!posregion) {
+      std::map<const uint32_t*, uint32_t*> made;
+      const uint64_t budget = A.avail() / 4;
+      uint64_t used = 0;
+      for (auto& pd : pairs) {
+        auto it = made.find(pd.bcand);
+        if (it == made.end()) {
+          uint32_t* m = nullptr;
+          if (used + 4ull * n <= budget && (m = (uint32_t*)A.alloc(4ull * n))) {
+            used += 4ull * n;
+            k_bscatter_pos<<<bgrid(c, pd.bsize, FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(pd.bcand, pd.bsize, m);
+          }
+          it = made.emplace(pd.bcand, m).first;
+        }
+        pd.bpos = it->second;
+      }
     }
     // sources: every (query, class a) with its targets, one sweep each
     std::vector<BSrc> srcs;
@@ -1658,18 +1673,18 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
       FSIG_HIP(c, hipGetLastError());
       if ((rc = read_u64(c, dseg, ch_seg.data(), nch + 1))) return rc;
     }
-    auto build_bits = [&](size_t j) -> int {
+    auto build_bits = [&](size_t j, bool with_pos) -> int {
       if (!use_bits) return FSIG_OK;
       k_fill_u32<<<bgrid(c, chunk_words[j], FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(region, chunk_words[j], 0);
       const size_t nt = bt_lo[j + 1] - bt_lo[j];
       if (nt && bt_total[j])
         k_bset_bits<<<bgrid(c, bt_total[j], FSIG_BLOCK), FSIG_BLOCK, 0, c->stream>>>(dbt + bt_lo[j], (uint32_t)nt,
-                                                                                     bt_total[j]);
+                                                                                     bt_total[j], with_pos);
       FSIG_HIP(c, hipGetLastError());
       return FSIG_OK;
     };
     for (size_t j = 0; j < nch; j++) {
-      if ((rc = build_bits(j))) return rc;
+      if ((rc = build_bits(j, false))) return rc;
       const uint64_t lo = ch_src[j], hi = ch_src[j + 1];
       if (hi > lo) {
         switch (RG) {
@@ -1687,7 +1702,9 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
     lst_all = (uint32_t*)A.alloc(4ull * std::max<uint64_t>(total_edges, 1));
     if (!lst_all) return set_err(c, FSIG_ENOMEM, "batch: arena exhausted for candidate lists");
     for (size_t j = 0; j < nch; j++) {
-      if (nch > 1 && (rc = build_bits(j))) return rc;  // one chunk: its bitmaps are still built
+      // one chunk: its bitmaps are still built (and its maps, if any, are the
+      // classes' own)
+      if (nch > 1 && (rc = build_bits(j, posregion != nullptr))) return rc;
       const uint64_t lo = ch_src[j], hi = ch_src[j + 1];
       if (hi > lo) {
         switch (RG) {
@@ -1949,6 +1966,11 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
       settled[i] = 1;
       settled_count[i] = rc2 ? 0 : hc[i] + drv.total();
       settled_rc[i] = rc2;  // over the context's work limit: refused
+      if (clk.on) {
+        fprintf(stderr, "[fsig batch]   resumed query %u (k %u): beam count %llu, %zu cut levels, status %d\n", i,
+                ks[i], (unsigned long long)hc[i], cut_rows[i].size(), rc2);
+        clk.mark("  resume");
+      }
     }
     return FSIG_OK;
   };
@@ -2075,6 +2097,7 @@ static int batch_impl(fsig_ctx* c, const fsig_graph* g, uint32_t nq, const uint3
     FSIG_HIP(c, hipEventRecord(c->ev_b, c->stream));
     FSIG_HIP(c, hipEventSynchronize(c->ev_b));
     FSIG_HIP(c, hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
+    clk.mark("lockstep levels");
     const int rc2 = any_bound ? resume_beamed() : FSIG_OK;
     A.release(mk);
     return rc2;

@@ -180,7 +180,8 @@ def test_batch_bitmaps_built_query_by_query(gpu_ctx):
     """On a graph whose batch of candidate bitmaps outgrows the last-level
     cache (4M vertices: 512 KB per target class, about 900 classes here),
     the candidate-edge build runs query by query with each query's bitmaps
-    rebuilt in one region: counts equal fsig_match's, bounded and exact."""
+    and vertex -> candidate index maps rebuilt in one region: counts equal
+    fsig_match's, bounded and exact."""
     g = F.HostGraph.rmat(22, 12_000_000, seed=41).set_labels("uniform", 16, seed=42)
     ctx = F.Context(0, 32 << 30)  # 128 class masks of 16 MB alone outgrow the shared test context
     dg = ctx.upload(g)
