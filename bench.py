@@ -425,7 +425,7 @@ def main():
     ap.add_argument("--workload", default="all", choices=["all", "c2", "c3", "c4", "c5", "ingest"])
     ap.add_argument("--qps", type=float, default=0.0, help="c5: arrival rate (0: the workload's)")
     ap.add_argument("--requests", type=int, default=0, help="c5: stream length (0: the workload's)")
-    ap.add_argument("--workers", type=int, default=8, help="c5: server worker contexts")
+    ap.add_argument("--workers", type=int, default=6, help="c5: server worker contexts")
     ap.add_argument("--max-batch", type=int, default=32, help="c5: most requests a worker batches")
     ap.add_argument("--c4-steps", type=int, default=5, help="all: timed batches of C4")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
