@@ -298,7 +298,9 @@ __global__ void k_bset_bits(const BBits* __restrict__ T, uint32_t nt, uint64_t t
 }
 
 // Candidates of more than kHubDeg neighbours are cut into segments
-// (k_seg_cedge, per pair) instead of one lane group.
+// (k_seg_cedge, per pair) instead of one lane group.  A group loads
+// kRowUnroll strides of a row before it tests any, so that many entry
+// gathers, then that many rounds of bitmap tests, are in flight at once.
 
 template <int G>
 __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_count_multi(const BSrc* __restrict__ S, uint32_t ns,
@@ -320,12 +322,20 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_count_multi(const BSrc* _
     if (e - s > kHubDeg) continue;  // k_seg_cedge
     if (gl == 0)
       for (uint32_t t = tm; t; t &= t - 1) acc[gw][__builtin_ctz(t)] = 0;
-    for (uint32_t j = s; j < e; j += G) {
-      const uint32_t i = j + gl;
-      const uint32_t m = i < e ? target_bits(sc, adj[i], tm) : 0u;
+    for (uint32_t j = s; j < e; j += kRowUnroll * G) {
+      uint32_t x[kRowUnroll], m[kRowUnroll];
+#pragma unroll
+      for (int u = 0; u < kRowUnroll; u++) {
+        const uint32_t i = j + u * G + gl;
+        x[u] = i < e ? adj[i] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < kRowUnroll; u++) m[u] = j + u * G + gl < e ? target_bits(sc, x[u], tm) : 0u;
       for (uint32_t t = tm; t; t &= t - 1) {
         const uint32_t b = __builtin_ctz(t);
-        const uint32_t got = (uint32_t)__popcll(__ballot((m >> b) & 1u) & gmask);
+        uint32_t got = 0;
+#pragma unroll
+        for (int u = 0; u < kRowUnroll; u++) got += (uint32_t)__popcll(__ballot((m[u] >> b) & 1u) & gmask);
         if (gl == 0) acc[gw][b] += got;
       }
     }
@@ -370,30 +380,36 @@ __global__ void __launch_bounds__(FSIG_BLOCK) k_bcedge_fill_multi(const BSrc* __
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (uint32_t j = s; j < e; j += G) {
-      const uint32_t i = j + gl;
-      uint32_t x = 0, m = 0;
-      if (i < e) {
-        x = adj[i];
-        m = target_bits(sc, x, tm);
+    for (uint32_t j = s; j < e; j += kRowUnroll * G) {
+      uint32_t x[kRowUnroll], m[kRowUnroll];
+#pragma unroll
+      for (int u = 0; u < kRowUnroll; u++) {
+        const uint32_t i = j + u * G + gl;
+        x[u] = i < e ? adj[i] : 0u;
       }
-      for (uint32_t t = tm; t; t &= t - 1) {
-        const uint32_t b = __builtin_ctz(t);
-        const bool hit = (m >> b) & 1u;
-        const uint64_t bal = __ballot(hit) & gmask;
-        if (bal == 0) continue;
-        const uint64_t a0 = at[gw][b];
-        if (hit) {
-          const BPair& pd = P[sc.pidx[b]];
-          lst_all[a0 + __popcll(bal & below)] = pd.bpos ? pd.bpos[x] : sorted_index(pd.bcand, pd.bsize, x);
+#pragma unroll
+      for (int u = 0; u < kRowUnroll; u++) m[u] = j + u * G + gl < e ? target_bits(sc, x[u], tm) : 0u;
+      // strides in order, each stride's targets in order: every list ascends
+#pragma unroll
+      for (int u = 0; u < kRowUnroll; u++) {
+        for (uint32_t t = tm; t; t &= t - 1) {
+          const uint32_t b = __builtin_ctz(t);
+          const bool hit = (m[u] >> b) & 1u;
+          const uint64_t bal = __ballot(hit) & gmask;
+          if (bal == 0) continue;
+          const uint64_t a0 = at[gw][b];
+          if (hit) {
+            const BPair& pd = P[sc.pidx[b]];
+            lst_all[a0 + __popcll(bal & below)] = pd.bpos ? pd.bpos[x[u]] : sorted_index(pd.bcand, pd.bsize, x[u]);
+          }
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          if (gl == 0) at[gw][b] = a0 + __popcll(bal);
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (gl == 0) at[gw][b] = a0 + __popcll(bal);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
     }
   }
