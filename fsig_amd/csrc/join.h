@@ -605,17 +605,21 @@ __device__ __forceinline__ uint64_t tail_row(const JoinLevel& L1, const CandEdge
 // which lives in dynamic LDS sized per launch (0 when no row of the launch
 // needs it).
 constexpr uint32_t kTailMax = 16;
+// Levels whose lists and conflicts all fit this many entries take a tail
+// with arrays of that size: a 64-group block then needs 13 KiB of LDS
+// instead of 25 KiB, so LDS no longer caps the waves per CU.
+constexpr uint32_t kTailSmall = 8;
 
 // N1: entries of the level-p arrays (MAXQ when a lockstep launch also keeps
-// its plain rows there, kTailMax for the plain tail, whose levels are
-// limited to that many backward lists and conflicts: less LDS per group,
-// more groups per CU).
-template <int G, int N1 = MAXQ>
+// its plain rows there; the plain tail's levels are limited to kTailMax
+// backward lists and conflicts: less LDS per group, more groups per CU);
+// N2: entries of the leaf-level arrays.
+template <int G, int N1 = MAXQ, int N2 = kTailMax>
 struct TailShared {
   static constexpr int GPB = FSIG_BLOCK / G;
   static constexpr uint32_t kTileWords = 4096 / GPB;  // at most 16 KiB of tiles per block
   uint32_t s1[GPB][N1], n1[GPB][N1], cf1[GPB][N1];
-  uint32_t s2[GPB][kTailMax], n2[GPB][kTailMax], cf2[GPB][kTailMax];
+  uint32_t s2[GPB][N2], n2[GPB][N2], cf2[GPB][N2];
   uint32_t slot[GPB][G];
   __device__ TailLds at(uint32_t gw, uint32_t* tile, uint32_t tw) {
     return TailLds{s1[gw], n1[gw], cf1[gw], s2[gw], n2[gw], cf2[gw], slot[gw], tile + gw * tw, tw};
@@ -631,12 +635,15 @@ struct TailIO {
   uint32_t tile_words;        // shared-intersection tile per group (dynamic LDS)
 };
 
-template <int G>
-__global__ void __launch_bounds__(FSIG_BLOCK) k_join_tail(JoinLevel L1, CandEdgeSrc S1, JoinLevel L2,
-                                                          CandEdgeSrc S2, TailIO io) {
+// N: entries of the per-group list and conflict arrays of both levels.  At
+// most 72 VGPRs, so seven waves per SIMD fit where LDS allows (the tail is
+// bound by dependent gathers; eight waves need spills and run slower).
+template <int G, int N>
+__global__ void __launch_bounds__(FSIG_BLOCK) __attribute__((amdgpu_waves_per_eu(7, 8)))
+k_join_tail(JoinLevel L1, CandEdgeSrc S1, JoinLevel L2, CandEdgeSrc S2, TailIO io) {
   constexpr int GPB = FSIG_BLOCK / G;
   constexpr int NR = (MAXQ + G - 1) / G;
-  __shared__ TailShared<G, kTailMax> sh;
+  __shared__ TailShared<G, N, N> sh;
   extern __shared__ uint32_t tail_tile[];
   const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gw = threadIdx.x / G;
   const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane - gl));
@@ -707,13 +714,18 @@ inline int launch_join_tail(fsig_ctx* c, uint32_t G, const JoinLevel& L1, const 
   if (blocks >= 64) blocks &= ~7ull;
   const uint32_t grid = (uint32_t)std::max<uint64_t>(blocks, 1);
   const size_t lds = (size_t)io.tile_words * gpb * sizeof(uint32_t);
+  const bool small = L1.nb <= kTailSmall && L1.ncf <= kTailSmall && L2.nb <= kTailSmall && L2.ncf <= kTailSmall;
+#define FSIG_TAIL(GV)                                                                       \
+  (small ? k_join_tail<GV, kTailSmall><<<grid, FSIG_BLOCK, lds, c->stream>>>(L1, S1, L2, S2, io) \
+         : k_join_tail<GV, kTailMax><<<grid, FSIG_BLOCK, lds, c->stream>>>(L1, S1, L2, S2, io))
   switch (G) {
-    case 4: k_join_tail<4><<<grid, FSIG_BLOCK, lds, c->stream>>>(L1, S1, L2, S2, io); break;
-    case 8: k_join_tail<8><<<grid, FSIG_BLOCK, lds, c->stream>>>(L1, S1, L2, S2, io); break;
-    case 16: k_join_tail<16><<<grid, FSIG_BLOCK, lds, c->stream>>>(L1, S1, L2, S2, io); break;
-    case 32: k_join_tail<32><<<grid, FSIG_BLOCK, lds, c->stream>>>(L1, S1, L2, S2, io); break;
-    default: k_join_tail<64><<<grid, FSIG_BLOCK, lds, c->stream>>>(L1, S1, L2, S2, io); break;
+    case 4: FSIG_TAIL(4); break;
+    case 8: FSIG_TAIL(8); break;
+    case 16: FSIG_TAIL(16); break;
+    case 32: FSIG_TAIL(32); break;
+    default: FSIG_TAIL(64); break;
   }
+#undef FSIG_TAIL
   FSIG_HIP(c, hipGetLastError());
   return FSIG_OK;
 }
