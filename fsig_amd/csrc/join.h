@@ -637,9 +637,11 @@ struct TailIO {
 
 // N: entries of the per-group list and conflict arrays of both levels.  At
 // most 72 VGPRs, so seven waves per SIMD fit where LDS allows (the tail is
-// bound by dependent gathers; eight waves need spills and run slower).
+// bound by dependent gathers; eight waves need spills and run slower); 64
+// four-lane groups with 16-entry arrays fit six per SIMD by their LDS.
 template <int G, int N>
-__global__ void __launch_bounds__(FSIG_BLOCK) __attribute__((amdgpu_waves_per_eu(7, 8)))
+__global__ void __launch_bounds__(FSIG_BLOCK)
+    __attribute__((amdgpu_waves_per_eu(G == 4 && N > (int)kTailSmall ? 6 : 7, 8)))
 k_join_tail(JoinLevel L1, CandEdgeSrc S1, JoinLevel L2, CandEdgeSrc S2, TailIO io) {
   constexpr int GPB = FSIG_BLOCK / G;
   constexpr int NR = (MAXQ + G - 1) / G;
