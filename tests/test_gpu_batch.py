@@ -199,38 +199,41 @@ def test_batch_bitmaps_built_query_by_query(gpu_ctx):
 
 
 def test_bounded_work_limit_sweep(gpu_ctx):
-    """Under work limits from 1 to 2^30 rows, a bounded count (fsig_match and
-    the bounded batch with a 16-row beam, whose resume counts the cut rows
-    under the limit) is min(count, bound) or refused with FSIG_ELIMIT: the
+    """Under work limits from 1 to 2^30 rows, a bounded count (fsig_match with
+    alg 1 and alg 2, and the bounded batch with a 16-row beam, whose resume
+    counts the cut rows under the limit) is min(count, bound) or refused with
+    FSIG_ELIMIT: the
     last window before the limit is clipped to the rows it leaves, so limits
     that fall inside a window run to the bound or refuse, never anything
     else.  The smallest limits refuse, the largest admit."""
     g = F.HostGraph.chung_lu(20000, 160000, 2.2, 99).set_labels("uniform", 2, seed=100)
-    dg = gpu_ctx.upload(g)
+    dg = gpu_ctx.upload(g, pcsr=True)
     q = F.Query(6, [0, 1, 0, 1, 0, 1], [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5)])
     bound = 10 ** 7
     want = F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT, max_embeddings=bound).count
     assert want == bound  # the bound is reached: the run stops early
-    plain, batch = [], []
+    plain, gsi, batch = [], [], []
     try:
         gpu_ctx.set_tune(beam_log2=4)
         for j in range(31):
             gpu_ctx.set_work_limit(1 << j)
-            try:
-                plain.append(F.match(dg, q, F.ALG_MMATCH, F.MODE_COUNT, max_embeddings=bound).count)
-            except F.FsigError as e:
-                assert e.code == F.FSIG_ELIMIT
-                plain.append(None)
+            for alg, out in ((F.ALG_MMATCH, plain), (F.ALG_GSI, gsi)):
+                try:
+                    out.append(F.match(dg, q, alg, F.MODE_COUNT, max_embeddings=bound).count)
+                except F.FsigError as e:
+                    assert e.code == F.FSIG_ELIMIT
+                    out.append(None)
             counts, _, st = F.match_batch(dg, [q], bounds=[bound], status=True)
             assert st[0] in (0, F.FSIG_ELIMIT)
             batch.append(int(counts[0]) if st[0] == 0 else None)
     finally:
         gpu_ctx.set_work_limit(0)
         gpu_ctx.set_tune()
-    for got in (plain, batch):
+    for got in (plain, gsi, batch):
         assert got[0] is None and got[-1] == want
         assert all(c in (None, want) for c in got)
-    print("admitted from limit 2^%d (plain), 2^%d (batch)" % (plain.index(want), batch.index(want)), flush=True)
+    print("admitted from limit 2^%d (alg 1), 2^%d (alg 2), 2^%d (batch)"
+          % (plain.index(want), gsi.index(want), batch.index(want)), flush=True)
     dg.free()
 
 
