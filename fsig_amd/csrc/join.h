@@ -1000,15 +1000,28 @@ class JoinDriver {
   static constexpr uint64_t kGrowMax = 1ull << 28;
   uint64_t chunk_[MAXQ + 1];
   uint64_t window_[MAXQ + 1];
+  // Embeddings per level-p row over the last chunk of that level that found
+  // any (0: none seen yet).  Near the bound, a chunk or window takes 1.25
+  // times the rows the remaining bound needs at that yield, not the doubled
+  // size: the rows beyond would be counted or written only to be cut off.
+  double yield_[MAXQ + 1] = {};
+
+  uint64_t need_rows(uint32_t p) const {
+    if (yield_[p] <= 0) return ~0ull;
+    const double r = 1.25 * (double)remaining() / yield_[p];
+    return r >= (double)kGrowMax ? kGrowMax : std::max<uint64_t>(kGrow0, (uint64_t)r + 1);
+  }
 
   int expand(uint32_t p, const uint32_t* rows, uint64_t R) {
     if (R == 0) return FSIG_OK;
     if (p == k_ || !bounded()) return expand_rows(p, rows, R);
     for (uint64_t r0 = 0; r0 < R && remaining() > 0;) {
-      const uint64_t n = std::min(chunk_[p], R - r0);
+      const uint64_t n = std::min(std::min(chunk_[p], R - r0), need_rows(p));
       chunk_[p] = std::min(chunk_[p] * 2, kGrowMax);
+      const uint64_t before = total_;
       const int rc = expand_rows(p, rows + r0 * p, n);
       if (rc) return rc;
+      if (total_ > before) yield_[p] = (double)(total_ - before) / (double)n;
       r0 += n;
     }
     return FSIG_OK;
@@ -1080,7 +1093,7 @@ class JoinDriver {
     const bool grow = bounded();
     uint64_t wsize;
     if (grow) {
-      wsize = std::min(cap, window_[p]);
+      wsize = std::min(std::min(cap, window_[p]), need_rows(p + 1));
     } else {
       if (limit && !last && rows_ + T > limit) return set_err(c_, FSIG_ELIMIT, "join: work limit exceeded");
       rows_ += last ? 0 : T;
@@ -1102,7 +1115,7 @@ class JoinDriver {
         }
         rows_ += last ? 0 : hi - lo;
         window_[p] = std::min(window_[p] * 2, kGrowMax);
-        wsize = std::min(cap, window_[p]);
+        wsize = std::min(std::min(cap, window_[p]), need_rows(p + 1));
       }
       uint64_t win[2] = {0, R};
       if (split) {  // parent rows of this window, found on the device
