@@ -59,3 +59,23 @@ def test_ingest_workload_line(monkeypatch, capsys):
     if O.ref_available():
         assert line["cpu_baseline"]["kind"] == "reference"
         assert set(line["stages"]["reference_c2"]["seconds"]) == {"read", "pcsr", "csr", "signature"}
+
+
+def test_roofline_traffic_matches_the_committed_profiles():
+    """The traffic bench.py reports for the dominant kernel comes from the
+    committed FETCH_SIZE / WRITE_SIZE summary: the fused-tail family is
+    there with positive bytes, and every kernel instance the summary names
+    appears in the committed C2 alg-1 kernel trace (the two passes profiled
+    the same kernels)."""
+    import bench
+    rec = json.load(open(bench.COUNTERS))
+    tail = bench.KERNEL_OF_FAMILY["join_tail"]
+    assert bench.measured_traffic(tail) > 0
+    cal = rec["calibration"]
+    assert abs(cal["fetch_factor"] - 2.0) < 0.01 and abs(cal["write_factor"] - 1.0) < 0.01
+    trace = open(os.path.join(ROOT, "profiles", "c2_alg1_kernel_trace.txt")).read()
+    for fam, k in rec["kernels"].items():
+        assert k["dispatches"] > 0 and k["fetch_bytes"] >= 0 and k["write_bytes"] >= 0, fam
+        for inst in k["instances"]:
+            assert inst in trace, (fam, inst)
+    assert bench.measured_traffic("no such kernel") is None
